@@ -1,0 +1,116 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Runs only where the reference tree exists (it compiles the reference's own
+sources with `make -C oracle ref` into oracle/_ref/, never copying them):
+
+* ref_hist_cfg3_e15_n{10000,100000}.sim -- hist.sim written by the reference's
+  dpm_Simulate (config index 3, 15 MeV e-, 1 mm voxels, mt19937(123)) on the
+  tables of dpm_tablegen configuration 3;
+* samplers_cfg3.npz -- reference sampler outputs (SimSBTables, SimMollerTables,
+  SimKNTables, SimGSTables) and table interpolations for seeded queries whose
+  uniforms lie on the 2^-24 lattice, as exact float64 values;
+* tables_cfg3.sha256 -- digest of the table files these were computed on.
+"""
+import hashlib
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "dpm-g4cpp_amd", "tablegen"))
+import dpm_tablegen  # noqa: E402
+
+TABLE_FILES = ("mat.dat", "el_itr1mfp.dat", "el_scatStrength.dat", "el_GSDtrData.dat", "eloss_rdedx.dat",
+               "imfp_brem.dat", "imfp_moller.dat", "ioni_MollerDtrData.dat", "brem_SBDtrData.dat",
+               "imfp_globalMax.dat", "imfp_total.dat", "imfp_compton.dat", "imfp_pairp.dat",
+               "compton_KNDtrData.dat")
+
+
+def tables_digest(d):
+    h = hashlib.sha256()
+    for f in TABLE_FILES:
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(f.encode())
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def lattice(rng, n):
+    k = rng.integers(0, 1 << 24, size=n, dtype=np.int64)
+    return k, k.astype(np.float64) / float(1 << 24)
+
+
+def main():
+    datadir = os.path.join(ROOT, "_work", "golden_tables_cfg3")
+    dpm_tablegen.generate(datadir, 3, verbose=False)
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
+    probe = os.path.join(ROOT, "oracle", "_ref", "ref_probe")
+    sim = os.path.join(ROOT, "oracle", "_ref", "dpm_Simulate_ref")
+    with open(os.path.join(HERE, "tables_cfg3.sha256"), "w") as f:
+        f.write(tables_digest(datadir) + "\n")
+
+    for n in (10000, 100000):
+        out = os.path.join(HERE, "ref_hist_cfg3_e15_n%d.sim" % n)
+        subprocess.check_call([sim, "-p", "e-", "-e", "15", "-n", str(n), "-d", datadir, "-b", "1.0", "-c", "3",
+                               "-o", out], stdout=subprocess.DEVNULL)
+
+    rng = np.random.default_rng(20240601)
+    n = 1000
+    fix = {}
+    queries = []
+
+    # samplers: energies inside each table's valid range (the reference
+    # samplers assume cut < E < Emax), plus lattice edge uniforms
+    def energies(lo, hi, m):
+        e = np.exp(rng.uniform(np.log(lo), np.log(hi), size=m))
+        return np.concatenate([e, [lo * 1.0001, hi * 0.9999, 15.0, 1.0]])
+
+    edge = np.array([0, 1, (1 << 23), (1 << 24) - 1], dtype=np.int64)
+    for kind, lo, hi, nr in (("sb", 0.0501, 20.99, 3), ("mo", 0.4001, 20.99, 3), ("kn", 0.0501, 20.99, 3),
+                             ("gs", 0.2001, 20.99, 2)):
+        e = energies(lo, hi, n)
+        m = e.size
+        ks = []
+        for j in range(nr):
+            k, _ = lattice(rng, m)
+            k[:edge.size] = edge[np.arange(edge.size) % edge.size] if j == nr - 1 else k[:edge.size]
+            ks.append(k)
+        ks = np.array(ks).T
+        imat = rng.integers(0, 3, size=m) if kind == "sb" else np.zeros(m, dtype=np.int64)
+        fix[kind + "_e"] = e
+        fix[kind + "_k"] = ks
+        fix[kind + "_imat"] = imat
+        for i in range(m):
+            r = ["%.17g" % (x / float(1 << 24)) for x in ks[i]]
+            if kind == "sb":
+                queries.append("sb %.17g %d %s" % (e[i], imat[i], " ".join(r)))
+            else:
+                queries.append("%s %.17g %s" % (kind, e[i], " ".join(r)))
+    # interpolations, including both clamps
+    ev_e = np.concatenate([np.exp(rng.uniform(np.log(0.04), np.log(25.0), size=400)), [0.001, 0.198, 21.0, 30.0]])
+    ev_rows = []
+    for which in range(9):
+        for im in (range(3) if which in (0, 1, 2, 6, 7, 8) else (0,)):
+            for x in ev_e:
+                ev_rows.append((which, x, im))
+                queries.append("ev %d %.17g %d" % (which, x, im))
+    fix["ev_rows"] = np.array([(w, x, im) for (w, x, im) in ev_rows])
+    res = subprocess.run([probe, datadir], input="\n".join(queries) + "\n", capture_output=True, text=True,
+                         check=True).stdout.split()
+    vals = np.array([float.fromhex(v) for v in res])
+    off = 0
+    for kind in ("sb", "mo", "kn", "gs"):
+        m = fix[kind + "_e"].size
+        fix[kind + "_ref"] = vals[off:off + m]
+        off += m
+    fix["ev_ref"] = vals[off:]
+    np.savez_compressed(os.path.join(HERE, "samplers_cfg3.npz"), **fix)
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()
