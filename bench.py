@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X DPM engine on its headline workload.
+
+Workload (one "step"): 1e7 photon histories in a homogeneous water cube of
+150^3 voxels of 2 mm (density 1.0), parallel 10x10 cm^2 field on the -z face,
+energies uniform in [0.5, 20] MeV, FP32 transport with an FP64 tally split in
+100 history batches (per-voxel sum and sum of squares). Inputs (tables,
+phantom) are resident on the GPU before the timed region; the timed region
+covers the whole transport of the step and the return of the dose.
+
+Printed: one JSON line with histories/s, the roofline of the dominant kernel
+(algorithmic bytes from the engine's event counters over that kernel's device
+time measured with HIP events on its stream), the 1 %/1 mm gamma pass rate of
+the FP32 dose against an FP64-transport GPU run of the same histories, and a
+CPU baseline (the FP64 oracle on this host's cores, a bounded sample).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dpm-g4cpp_amd", "python"))
+sys.path.insert(0, os.path.join(ROOT, "dpm-g4cpp_amd", "tablegen"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per event (FP32 transport): voxel visit = u8 material +
+# f32 density; deposit = FP64 read-modify-write; alias sample = 3 entries of
+# 16 B; GS sample = 2 entries of 16 B + a(E) and E-grid pairs; secondary =
+# one 44 B queue record written and read back
+BYTES = dict(voxel_visits=5, deposits=16, sb=48, moller=48, kn=48, gs=48, secondaries=88)
+
+
+def algorithmic_bytes(c):
+    return sum(BYTES[k] * c[k] for k in BYTES)
+
+
+def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
+    import oracle_lib  # the checker's CLI, timed as the CPU baseline
+    cli = oracle_lib.CLI_PATH
+    if not os.path.exists(cli):
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    args = [cli, "--data", tables0, "--mode", "philox", "--geom", "voxel", "--nx", str(nx), "--lbox", str(lbox),
+            "--ptype", "0", "--emin", str(emin), "--emax", str(emax), "--field", str(field)]
+    # calibrate on one core
+    out = subprocess.run(args + ["--n", "2000", "--hist-begin", "0"], capture_output=True, text=True, check=True)
+    rate1 = json.loads(out.stdout.strip().splitlines()[-1])["hist_per_s"]
+    n_each = max(2000, int(rate1 * seconds_target))
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen(args + ["--n", str(n_each), "--hist-begin", str(1000000 + i * n_each)],
+                              stdout=subprocess.PIPE, text=True) for i in range(cores)]
+    outs = [p.communicate()[0] for p in procs]
+    wall = time.perf_counter() - t0
+    if any(p.returncode != 0 for p in procs):
+        return None
+    per = [json.loads(o.strip().splitlines()[-1]) for o in outs]
+    return {"value": cores * n_each / wall, "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": "%d processes x %d histories of the same workload (FP64 oracle, Philox mode), wall %.1f s; "
+                      "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories", type=float, default=1e7)
+    ap.add_argument("--nbatch", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gamma", action="store_true")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; the multi-GPU split (history ranges per GPU, one FP64 "
+              "all-reduce of the dose) is not built, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import __graft_entry__
+    import dpm_amd
+    import gamma
+
+    tables0 = __graft_entry__.tables_dir(0)
+    nx, lbox, field, emin, emax = 150, 2.0, 100.0, 0.5, 20.0
+    nhist = int(args.histories)
+    t = dpm_amd.Tables(tables0)
+    mat = np.zeros((nx, nx, nx), dtype=np.uint8)
+    rho = np.ones((nx, nx, nx), dtype=np.float32)
+    ph = dpm_amd.Phantom.voxel(t, mat, rho, lbox)
+
+    def step(precision=dpm_amd.DPM_FP32, seed=1):
+        return dpm_amd.simulate(t, ph, nhist, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=seed,
+                                precision=precision, nbatch=args.nbatch, chunk=args.chunk,
+                                waves_per_cu=args.waves_per_cu)
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dose, sum2, st, _ = step()
+    wall = time.perf_counter() - t0
+    value = args.steps * nhist / wall
+
+    # roofline of the dominant kernel (device time from HIP events on the run stream)
+    ms_ph, ms_el = st["kernel_ms"][1], st["kernel_ms"][2]
+    dom = "electrons" if ms_el >= ms_ph else "photons"
+    cnt = st["counters_electron"] if dom == "electrons" else st["counters_photon"]
+    launches = st["kernel_calls"][2 if dom == "electrons" else 1]
+    ms_dom = ms_el if dom == "electrons" else ms_ph
+    bytes_per_launch = algorithmic_bytes(cnt) / max(1, launches)
+    avg_ms = ms_dom / max(1, launches)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "traffic_k_electrons_water150_photons.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    result = {
+        "metric": "histories/s", "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "config2: 150^3 water cube, 2 mm voxels, 10x10 cm2 parallel photon field, "
+                               "E ~ U(0.5, 20) MeV, %d histories per step, FP32 transport, FP64 tally, %d batches"
+                               % (nhist, args.nbatch),
+                   "histories": nhist, "voxels": nx ** 3, "nbatch": args.nbatch},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches},
+        "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
+        "device_seconds_per_step": st["seconds"],
+        "counters": st["counters"],
+    }
+    if not args.no_gamma:
+        d64, _, st64, _ = step(precision=dpm_amd.DPM_FP64)
+        pr = gamma.gamma_pass_rate_3d(d64, dose, spacing_mm=lbox, dd=0.01, dta_mm=1.0, cutoff=0.1)
+        m = d64 >= 0.1 * d64.max()
+        result["gamma_fp32_vs_fp64"] = {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max",
+                                        "max_rel_dose_diff_above_cutoff":
+                                            float(np.max(np.abs(dose[m] - d64[m]) / d64.max())),
+                                        "fp64_histories_per_s": nhist / st64["seconds"]}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1034 @@
+// dpm_capi.hip -- host engine and C ABI of the MI355X DPM transport library.
+//
+// Owns the device table packs, phantoms and queue workspaces, and drives the
+// transport kernels (dpm_kernels.hip) chunk by chunk. See include/dpm_amd.h
+// for the entry points and the reference interfaces they replace.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dpm_amd.h"
+#include "dpm_engine.hpp"
+#include "../host/SimData.hh"
+
+using namespace dpm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return set_error(DPM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// natural cubic spline coefficients, same recurrences and operation order as
+// SimDataSpline::SetUp (SimDataSpline.cc:33-67)
+// ---------------------------------------------------------------------------
+void spline_coefficients(const double* x, const double* y, int n, std::vector<double>& B, std::vector<double>& C,
+                         std::vector<double>& D) {
+  B.assign(n, 0.0); C.assign(n, 0.0); D.assign(n, 0.0);
+  const int last = n - 1;
+  double s = 0.0, r = 0.0;
+  for (int m = 0; m < last; ++m) {
+    D[m] = x[m + 1] - x[m];
+    r = (y[m + 1] - y[m]) / D[m];
+    C[m] = r - s;
+    s = r;
+  }
+  r = 0.0; s = 0.0;
+  C[0] = 0.0; C[last] = 0.0;
+  for (int m = 1; m < last; ++m) {
+    C[m] = C[m] + r * C[m - 1];
+    B[m] = 2.0 * (x[m - 1] - x[m + 1]) - r * s;
+    s = D[m];
+    r = s / B[m];
+  }
+  for (int m = last - 1; m >= 1; --m) C[m] = (D[m] * C[m + 1] - C[m]) / B[m];
+  for (int m = 0; m < last; ++m) {
+    s = D[m];
+    r = C[m + 1] - C[m];
+    D[m] = r / s;
+    C[m] = 3.0 * C[m];
+    B[m] = (y[m + 1] - y[m]) / s - (C[m] + r) * s;
+  }
+}
+
+// integer keep-threshold of an alias entry: (w < f * 2^-24) <=> (f >= thr)
+uint32_t alias_threshold(double w) {
+  if (!(w >= 0.0)) return 0u;
+  const double t = std::floor(std::ldexp(w, 24)) + 1.0;
+  return t > 16777216.0 ? 16777216u : (uint32_t)t;
+}
+
+// byte buffer assembled on the host, uploaded once
+struct Blob {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  size_t add(const T* p, size_t n) {
+    size_t off = (bytes.size() + 255) & ~(size_t)255;
+    bytes.resize(off + n * sizeof(T));
+    std::memcpy(bytes.data() + off, p, n * sizeof(T));
+    return off;
+  }
+};
+
+template <typename R>
+struct PackBuilder {
+  Blob blob;
+  // offsets of every array; patched into pointers after the upload
+  size_t o_itr1, o_dedx, o_brem, o_moller, o_k1, o_gmax, o_ptot, o_pcomp, o_ppair, o_sb, o_mo, o_kn, o_gs, o_gse,
+      o_gst;
+  Tables<R> T;
+
+  size_t add_spline(const dpm_spline_desc* per, int nm, SplineTab<R>& t) {
+    const int n = per[0].n;
+    std::vector<SplineNode<R>> nodes((size_t)nm * n);
+    for (int im = 0; im < nm; ++im) {
+      std::vector<double> B, C, D;
+      spline_coefficients(per[im].x, per[im].y, n, B, C, D);
+      for (int i = 0; i < n; ++i) {
+        SplineNode<R>& nd = nodes[(size_t)im * n + i];
+        nd.x = (R)per[im].x[i]; nd.y = (R)per[im].y[i];
+        nd.b = (R)B[i]; nd.c = (R)C[i]; nd.d = (R)D[i];
+        nd.pad0 = nd.pad1 = nd.pad2 = (R)0;
+      }
+    }
+    t.n = n;
+    const double lmin = std::log(per[0].x[0]);
+    t.logMinX = (R)lmin;
+    t.invLogDelta = (R)(1. / (std::log(per[0].x[1]) - lmin));
+    t.emin = (R)per[0].x[0];
+    t.emaxm = (R)(per[0].x[n - 1] - 1.0E-6);
+    return blob.add(nodes.data(), nodes.size());
+  }
+  size_t add_linear(const dpm_linear_desc* per, int nm, LinTab<R>& t) {
+    const int n = per[0].n;
+    std::vector<R> xy((size_t)nm * n * 2);
+    for (int im = 0; im < nm; ++im)
+      for (int i = 0; i < n; ++i) {
+        xy[2 * ((size_t)im * n + i)] = (R)per[im].x[i];
+        xy[2 * ((size_t)im * n + i) + 1] = (R)per[im].y[i];
+      }
+    t.n = n;
+    t.minX = (R)per[0].x[0];
+    t.invDelta = (R)(1. / (per[0].x[1] - per[0].x[0]));
+    t.emin = (R)per[0].x[0];
+    t.emaxm = (R)(per[0].x[n - 1] - 1.0E-6);
+    return blob.add(xy.data(), xy.size());
+  }
+  size_t add_alias(const dpm_alias_desc& d, AliasSet<R>& s) {
+    const size_t n = (size_t)d.nmat * d.ne * d.ns;
+    std::vector<AliasEntry<R>> e(n);
+    for (size_t k = 0; k < n; ++k) {
+      e[k].x = (R)d.x[k];
+      e[k].y = (R)d.y[k];
+      e[k].thr = alias_threshold(d.w[k]);
+      // an alias is followed only when frac >= thr; keep it inside the table
+      e[k].alias = (uint32_t)std::min(std::max(d.alias[k], 0), d.ns - 2);
+    }
+    s.ne = d.ne;
+    s.ns = d.ns;
+    s.minE = (R)d.energy[0];
+    s.logMinE_d = std::log(d.energy[0]);
+    s.invLogDelta_d = 1. / (std::log(d.energy[1]) - s.logMinE_d);
+    s.logMinE = (R)s.logMinE_d;
+    s.invLogDelta = (R)s.invLogDelta_d;
+    return blob.add(e.data(), n);
+  }
+};
+
+bool same_grid(const double* a, const double* b, int n) {
+  for (int i = 0; i < n; ++i)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+
+int validate(const dpm_table_desc* d) {
+  if (!d) return set_error(DPM_E_INVALID, "null table descriptor");
+  if (d->nmat < 1 || d->nmat > kMaxMat)
+    return set_error(DPM_E_INVALID, "number of materials must be in [1, " + std::to_string(kMaxMat) + "]");
+  const dpm_spline_desc* sp[3] = {d->itr1, d->dedx, d->brem};
+  const dpm_linear_desc* ln[3] = {d->ptotal, d->pcompton, d->ppair};
+  for (int k = 0; k < 3; ++k) {
+    if (!sp[k] || !ln[k]) return set_error(DPM_E_INVALID, "missing per-material table");
+    for (int im = 0; im < d->nmat; ++im) {
+      if (sp[k][im].n < 2 || sp[k][im].n != sp[k][0].n || !sp[k][im].x || !sp[k][im].y)
+        return set_error(DPM_E_INVALID, "per-material spline tables must share one grid size >= 2");
+      if (!same_grid(sp[k][im].x, sp[k][0].x, sp[k][0].n))
+        return set_error(DPM_E_INVALID, "per-material spline tables must share one energy grid");
+      if (ln[k][im].n < 2 || ln[k][im].n != ln[k][0].n || !ln[k][im].x || !ln[k][im].y)
+        return set_error(DPM_E_INVALID, "per-material photon tables must share one grid size >= 2");
+      if (!same_grid(ln[k][im].x, ln[k][0].x, ln[k][0].n))
+        return set_error(DPM_E_INVALID, "per-material photon tables must share one energy grid");
+    }
+  }
+  if (d->moller.n < 2 || d->k1.n < 2 || d->gmax.n < 2) return set_error(DPM_E_INVALID, "reference table too short");
+  const dpm_alias_desc* al[3] = {&d->sb, &d->moller_tab, &d->kn};
+  for (int k = 0; k < 3; ++k) {
+    if (al[k]->ne < 2 || al[k]->ns < 2 || al[k]->ns > 256 || !al[k]->x || !al[k]->energy)
+      return set_error(DPM_E_INVALID, "malformed alias table set");
+    const size_t n = (size_t)al[k]->nmat * al[k]->ne * al[k]->ns;
+    for (size_t i = 0; i < n; ++i)
+      if (al[k]->alias[i] < 0 || al[k]->alias[i] >= al[k]->ns)
+        return set_error(DPM_E_INVALID, "alias index out of range");
+  }
+  if (d->sb.nmat != d->nmat) return set_error(DPM_E_INVALID, "SB tables must cover every material");
+  if (d->gs.ne < 2 || d->gs.ns < 2) return set_error(DPM_E_INVALID, "malformed GS tables");
+  return DPM_OK;
+}
+
+template <typename R>
+int build_pack(const dpm_table_desc* d, Tables<R>& T, void** dev) {
+  PackBuilder<R> pb;
+  std::memset(&T, 0, sizeof(T));
+  const size_t o_itr1 = pb.add_spline(d->itr1, d->nmat, T.itr1);
+  const size_t o_dedx = pb.add_spline(d->dedx, d->nmat, T.dedx);
+  const size_t o_brem = pb.add_spline(d->brem, d->nmat, T.brem);
+  const size_t o_mol = pb.add_spline(&d->moller, 1, T.moller);
+  const size_t o_k1 = pb.add_spline(&d->k1, 1, T.k1);
+  const size_t o_gmax = pb.add_linear(&d->gmax, 1, T.gmax);
+  const size_t o_ptot = pb.add_linear(d->ptotal, d->nmat, T.ptot);
+  const size_t o_pcomp = pb.add_linear(d->pcompton, d->nmat, T.pcomp);
+  const size_t o_ppair = pb.add_linear(d->ppair, d->nmat, T.ppair);
+  const size_t o_sb = pb.add_alias(d->sb, T.sb);
+  const size_t o_mo = pb.add_alias(d->moller_tab, T.mo);
+  const size_t o_kn = pb.add_alias(d->kn, T.kn);
+  // GS
+  const dpm_gs_desc& g = d->gs;
+  std::vector<GSEntry<R>> ge((size_t)g.ne * g.ns);
+  for (size_t k = 0; k < ge.size(); ++k) {
+    ge[k].u = (R)g.u[k]; ge[k].a = (R)g.a[k]; ge[k].b = (R)g.b[k]; ge[k].pad = (R)0;
+  }
+  std::vector<R> gen(g.ne), gtr(g.ne);
+  for (int i = 0; i < g.ne; ++i) { gen[i] = (R)g.energy[i]; gtr[i] = (R)g.transf[i]; }
+  const size_t o_gs = pb.blob.add(ge.data(), ge.size());
+  const size_t o_gse = pb.blob.add(gen.data(), gen.size());
+  const size_t o_gst = pb.blob.add(gtr.data(), gtr.size());
+  T.gs.ne = g.ne;
+  T.gs.ns = g.ns;
+  T.gs.logMinE_d = std::log(g.energy[0]);
+  T.gs.invLogDelta_d = 1. / (std::log(g.energy[1]) - T.gs.logMinE_d);
+  T.gs.logMinE = (R)T.gs.logMinE_d;
+  T.gs.invLogDelta = (R)T.gs.invLogDelta_d;
+  T.gs.deltaCum = (R)(1.0 / (g.ns - 1));
+  for (int im = 0; im < d->nmat; ++im) {
+    T.rho[im] = (R)d->density[im];
+    T.mscale[im] = (R)d->moller_scale[im];
+  }
+  T.ecut = (R)d->electron_cut;
+  T.gcut = (R)d->gamma_cut;
+  T.nmat = d->nmat;
+  // upload and patch pointers
+  unsigned char* base = nullptr;
+  HIP_TRY(hipMalloc(&base, pb.blob.bytes.size()));
+  HIP_TRY(hipMemcpy(base, pb.blob.bytes.data(), pb.blob.bytes.size(), hipMemcpyHostToDevice));
+  T.itr1.node = (const SplineNode<R>*)(base + o_itr1);
+  T.dedx.node = (const SplineNode<R>*)(base + o_dedx);
+  T.brem.node = (const SplineNode<R>*)(base + o_brem);
+  T.moller.node = (const SplineNode<R>*)(base + o_mol);
+  T.k1.node = (const SplineNode<R>*)(base + o_k1);
+  T.gmax.xy = (const R*)(base + o_gmax);
+  T.ptot.xy = (const R*)(base + o_ptot);
+  T.pcomp.xy = (const R*)(base + o_pcomp);
+  T.ppair.xy = (const R*)(base + o_ppair);
+  T.sb.e = (const AliasEntry<R>*)(base + o_sb);
+  T.mo.e = (const AliasEntry<R>*)(base + o_mo);
+  T.kn.e = (const AliasEntry<R>*)(base + o_kn);
+  T.gs.e = (const GSEntry<R>*)(base + o_gs);
+  T.gs.egrid = (const R*)(base + o_gse);
+  T.gs.transf = (const R*)(base + o_gst);
+  *dev = base;
+  return DPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// queue workspace
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Workspace {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  int cap = 0;
+  QueueView<R> q[4];   // G0, G1, E0, E1
+
+  void release() {
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    bytes = 0;
+    cap = 0;
+  }
+  int ensure(int want, int* ctl) {
+    if (cap >= want && buf) {
+      for (int k = 0; k < 4; ++k) q[k].count = ctl + k;
+      return DPM_OK;
+    }
+    release();
+    const size_t per = (size_t)want * (7 * sizeof(R) + 4 * sizeof(uint32_t));
+    const size_t total = 4 * (per + 11 * 256);
+    if (hipMalloc(&buf, total) != hipSuccess) {
+      buf = nullptr;
+      return set_error(DPM_E_NOMEM, "cannot allocate particle queues of " + std::to_string(total >> 20) + " MiB");
+    }
+    bytes = total;
+    cap = want;
+    unsigned char* p = (unsigned char*)buf;
+    auto take = [&](size_t n) {
+      unsigned char* r = p;
+      p += (n + 255) & ~(size_t)255;
+      return r;
+    };
+    for (int k = 0; k < 4; ++k) {
+      QueueView<R>& v = q[k];
+      v.x = (R*)take(sizeof(R) * want); v.y = (R*)take(sizeof(R) * want); v.z = (R*)take(sizeof(R) * want);
+      v.u = (R*)take(sizeof(R) * want); v.v = (R*)take(sizeof(R) * want); v.w = (R*)take(sizeof(R) * want);
+      v.e = (R*)take(sizeof(R) * want);
+      v.hist = (uint32_t*)take(4 * (size_t)want); v.slo = (uint32_t*)take(4 * (size_t)want);
+      v.shi = (uint32_t*)take(4 * (size_t)want); v.tt = (uint32_t*)take(4 * (size_t)want);
+      v.cap = want;
+      v.count = ctl + k;
+    }
+    return DPM_OK;
+  }
+};
+
+// tally fold: per-batch rows -> sum, sum of squares; rows cleared
+__global__ void k_fold(double* rows, int nrows, int64_t nvox, double* sum, double* sum2) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nvox; v += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0, s2 = 0;
+    for (int r = 0; r < nrows; ++r) {
+      const double x = rows[(size_t)r * nvox + v];
+      if (x != 0.0) {
+        s += x;
+        s2 += x * x;
+        rows[(size_t)r * nvox + v] = 0.0;
+      }
+    }
+    if (s2 != 0.0) {
+      sum[v] += s;
+      sum2[v] += s2;
+    }
+  }
+}
+
+// per-row maximum of the beamlet tally (one block per row)
+__global__ void k_rowmax(const double* rows, int64_t nvox, double* rmax) {
+  __shared__ double red[256];
+  const double* r = rows + (size_t)blockIdx.x * nvox;
+  double m = 0;
+  for (int64_t v = threadIdx.x; v < nvox; v += blockDim.x) m = fmax(m, r[v]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rmax[blockIdx.x] = red[0];
+}
+
+// count (write == 0) or write (write == 1) the entries above threshold of each
+// row; one block per row, entries written in voxel order
+__global__ void k_csr(const double* rows, int64_t nvox, const double* rmax, double rel, const int64_t* rowptr,
+                      int64_t* counts, int32_t* col, double* val, int write) {
+  __shared__ int64_t base;
+  __shared__ int wsum[4];
+  const double* r = rows + (size_t)blockIdx.x * nvox;
+  const double thr = rel * rmax[blockIdx.x];
+  if (threadIdx.x == 0) base = write ? rowptr[blockIdx.x] : 0;
+  __syncthreads();
+  for (int64_t v0 = 0; v0 < nvox; v0 += blockDim.x) {
+    const int64_t v = v0 + threadIdx.x;
+    const bool keep = v < nvox && r[v] > thr && r[v] > 0.0;
+    const uint64_t b = __ballot(keep);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[wave] = __popcll(b);
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    const int rank = before + (int)__popcll(b & ((1ull << (threadIdx.x & 63)) - 1ull));
+    if (keep && write) {
+      col[base + rank] = (int32_t)v;
+      val[base + rank] = r[v];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (!write && threadIdx.x == 0) counts[blockIdx.x] = base;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct dpm_tables {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  int nmat = 0;
+  double ecut = 0, gcut = 0;
+  std::vector<double> rho;
+  Tables<float> tf;
+  Tables<double> td;
+  void* dev_f = nullptr;
+  void* dev_d = nullptr;
+  Workspace<float> wsf;
+  Workspace<double> wsd;
+  int* ctl = nullptr;        // [0..3] queue counts, [4] head, [5] overflow
+  StatsDev* stats = nullptr;
+  hipEvent_t ev[10] = {};
+  // CSR of the last beamlet run
+  int32_t* csr_col = nullptr;
+  double* csr_val = nullptr;
+  int64_t csr_nnz = 0;
+};
+
+struct dpm_phantom {
+  const dpm_tables* owner = nullptr;
+  int geom = GEOM_SLAB;
+  double lbox = 1.0;
+  int geom_index = 0;
+  int nx = 0, ny = 0, nz = 0;
+  uint8_t* d_mat = nullptr;
+  float* d_rho = nullptr;
+  double maj_scale = 1.0;
+  int nbins = 0;
+  int64_t nvox = 0;
+};
+
+namespace {
+
+template <typename R>
+void fill_geometry(const dpm_tables* t, const dpm_phantom* p, Geometry<R>& G) {
+  std::memset(&G, 0, sizeof(G));
+  G.L = (R)p->lbox;
+  G.invL = (R)(1. / p->lbox);
+  G.hL = (R)(0.5 * p->lbox);
+  G.geomIndex = p->geom_index;
+  G.nx = p->nx; G.ny = p->ny; G.nz = p->nz;
+  G.org[0] = (R)(-0.5 * p->nx * p->lbox);
+  G.org[1] = (R)(-0.5 * p->ny * p->lbox);
+  G.org[2] = (R)(-0.5 * p->nz * p->lbox);
+  G.vmat = p->d_mat;
+  G.vrho = p->d_rho;
+  for (int im = 0; im < t->nmat; ++im) G.rho[im] = (R)t->rho[im];
+}
+
+// device-time of a pair of events, 0 when unavailable
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+template <typename R>
+struct Launch;
+template <>
+struct Launch<float> {
+  static hipError_t source(const KParams<float>& P, const SourceParams& S, uint64_t h0, int n,
+                           const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
+    return launch_source_f32(P, S, h0, n, qe, qg, st);
+  }
+  static hipError_t photons(const KParams<float>& P, int g, int b, const QueueView<float>& in, int* h,
+                            const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
+    return launch_photons_f32(P, g, b, in, h, qe, qg, st);
+  }
+  static hipError_t electrons(const KParams<float>& P, int g, int b, const QueueView<float>& in, int* h,
+                              const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
+    return launch_electrons_f32(P, g, b, in, h, qe, qg, st);
+  }
+};
+template <>
+struct Launch<double> {
+  static hipError_t source(const KParams<double>& P, const SourceParams& S, uint64_t h0, int n,
+                           const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
+    return launch_source_f64(P, S, h0, n, qe, qg, st);
+  }
+  static hipError_t photons(const KParams<double>& P, int g, int b, const QueueView<double>& in, int* h,
+                            const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
+    return launch_photons_f64(P, g, b, in, h, qe, qg, st);
+  }
+  static hipError_t electrons(const KParams<double>& P, int g, int b, const QueueView<double>& in, int* h,
+                              const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
+    return launch_electrons_f64(P, g, b, in, h, qe, qg, st);
+  }
+};
+
+// boundaries of batch b: first history ceil(b*N/nb)
+uint64_t batch_start(uint64_t b, uint64_t n, uint64_t nb) {
+  const unsigned __int128 num = (unsigned __int128)b * n;
+  return (uint64_t)((num + nb - 1) / nb);
+}
+
+// Runs histories [0, nhist) of the source (offset by opt->hist_begin) and
+// leaves the tally on the device in `rows` (rows x nbins|nvox doubles).
+template <typename R>
+int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t nhist, const dpm_run_options* opt,
+                  Workspace<R>& ws, Tables<R>& T, double* d_rows, int nrows_per_chunk, double* d_steps,
+                  double* d_sum, double* d_sum2, double* d_hist_edep, dpm_stats* st) {
+  const hipStream_t s = t->stream;
+  KParams<R> P;
+  std::memset(&P, 0, sizeof(P));
+  P.T = T;
+  fill_geometry(t, p, P.G);
+  P.k0 = (uint32_t)opt->seed;
+  P.k1 = (uint32_t)(opt->seed >> 32);
+  P.majScale = (R)p->maj_scale;
+  P.tally = d_rows;
+  P.steps = d_steps;
+  P.nvox = p->nvox;
+  P.nbins = p->nbins;
+  P.hist_edep = d_hist_edep;
+  P.hist_base = opt->hist_begin;
+  P.stats = t->stats;
+  P.overflow = t->ctl + 5;
+
+  const int nbatch = (S.nbeamlet == 0 && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
+  int64_t chunk = opt->chunk > 0 ? opt->chunk : (int64_t)1 << 20;
+  if (chunk > ((int64_t)1 << 24)) chunk = (int64_t)1 << 24;
+  const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + 65536, (int64_t)1 << 30);
+  int rc = ws.ensure(cap, t->ctl);
+  if (rc != DPM_OK) return rc;
+  const int wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
+  const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
+  S.hist_begin = opt->hist_begin;
+  S.nhist_total = nhist;
+  S.nbatch = nbatch;
+  S.z0 = p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox;
+
+  HIP_TRY(hipEventRecord(t->ev[0], s));
+  uint64_t done = 0;
+  uint64_t next_batch = 0;
+  int rounds = 0;
+  float ms_src = 0, ms_ph = 0, ms_el = 0, ms_fold = 0;
+  int n_src = 0, n_ph = 0, n_el = 0, n_fold = 0;
+  while (done < nhist) {
+    uint64_t n = std::min<uint64_t>((uint64_t)chunk, nhist - done);
+    uint32_t bfirst = 0;
+    int nrows = 1;
+    if (nbatch > 0) {
+      // whole batches per chunk: at least one, as many as fit in `chunk` rows
+      const uint64_t b0 = next_batch;
+      uint64_t b1 = b0 + 1;
+      while (b1 < (uint64_t)nbatch && (int)(b1 - b0) < nrows_per_chunk &&
+             batch_start(b1 + 1, nhist, nbatch) - batch_start(b0, nhist, nbatch) <= (uint64_t)chunk)
+        ++b1;
+      n = batch_start(b1, nhist, nbatch) - batch_start(b0, nhist, nbatch);
+      bfirst = (uint32_t)b0;
+      nrows = (int)(b1 - b0);
+      next_batch = b1;
+    }
+    S.batch_first = bfirst;
+    HIP_TRY(hipMemsetAsync(t->ctl, 0, 8 * sizeof(int), s));
+    int cur = 0;
+    HIP_TRY(hipEventRecord(t->ev[1], s));
+    HIP_TRY(Launch<R>::source(P, S, opt->hist_begin + done, (int)n, ws.q[2 + cur], ws.q[0 + cur], s));
+    HIP_TRY(hipEventRecord(t->ev[2], s));
+    ++n_src;
+    bool first = true;
+    for (;;) {
+      const int nxt = cur ^ 1;
+      // clear next-round counts and the consumption head
+      HIP_TRY(hipMemsetAsync(t->ctl + nxt, 0, sizeof(int), s));
+      HIP_TRY(hipMemsetAsync(t->ctl + 2 + nxt, 0, sizeof(int), s));
+      HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
+      HIP_TRY(hipEventRecord(t->ev[3], s));
+      P.kind = 0;
+      HIP_TRY(Launch<R>::photons(P, p->geom, blocks, ws.q[0 + cur], t->ctl + 4, ws.q[2 + cur], ws.q[0 + nxt], s));
+      HIP_TRY(hipEventRecord(t->ev[4], s));
+      HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
+      HIP_TRY(hipEventRecord(t->ev[5], s));
+      P.kind = 1;
+      HIP_TRY(Launch<R>::electrons(P, p->geom, blocks, ws.q[2 + cur], t->ctl + 4, ws.q[2 + nxt], ws.q[0 + nxt], s));
+      HIP_TRY(hipEventRecord(t->ev[6], s));
+      int counts[6];
+      HIP_TRY(hipMemcpyAsync(counts, t->ctl, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (first) { ms_src += elapsed(t->ev[1], t->ev[2]); first = false; }
+      ms_ph += elapsed(t->ev[3], t->ev[4]);
+      ms_el += elapsed(t->ev[5], t->ev[6]);
+      ++n_ph; ++n_el;
+      ++rounds;
+      if (counts[5] != 0)
+        return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
+                                             "); use a smaller chunk");
+      if (counts[0 + nxt] == 0 && counts[2 + nxt] == 0) break;
+      cur = nxt;
+      if (rounds > 100000) return set_error(DPM_E_DEVICE, "transport did not drain");
+    }
+    if (nbatch > 0) {
+      HIP_TRY(hipEventRecord(t->ev[7], s));
+      hipLaunchKernelGGL(k_fold, dim3(std::max(1, t->num_cu * 8)), dim3(256), 0, s, d_rows, nrows, p->nvox, d_sum,
+                         d_sum2);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(t->ev[8], s));
+      HIP_TRY(hipEventSynchronize(t->ev[8]));
+      ms_fold += elapsed(t->ev[7], t->ev[8]);
+      ++n_fold;
+    }
+    done += n;
+  }
+  HIP_TRY(hipEventRecord(t->ev[9], s));
+  HIP_TRY(hipEventSynchronize(t->ev[9]));
+  if (st) {
+    st->seconds = elapsed(t->ev[0], t->ev[9]) * 1e-3;
+    st->rounds = rounds;
+    st->kernel_ms[0] = ms_src; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
+    st->kernel_calls[0] = n_src; st->kernel_calls[1] = n_ph; st->kernel_calls[2] = n_el; st->kernel_calls[3] = n_fold;
+  }
+  return DPM_OK;
+}
+
+int finish_stats(dpm_tables* t, dpm_stats* st) {
+  StatsDev h;
+  HIP_TRY(hipMemcpy(&h, t->stats, sizeof(h), hipMemcpyDeviceToHost));
+  if (st) {
+    for (int i = 0; i < DPM_NCNT; ++i) {
+      st->counters_photon[i] = h.cnt[0][i];
+      st->counters_electron[i] = h.cnt[1][i];
+      st->counters[i] = h.cnt[0][i] + h.cnt[1][i];
+    }
+    st->n_pair = h.cnt[0][kCntPair] + h.cnt[1][kCntPair];
+    st->n_annih = h.cnt[0][kCntAnnih] + h.cnt[1][kCntAnnih];
+    st->e_escaped = h.escaped;
+    st->hist_size = std::max(20, h.maxiz + 10);
+  }
+  return DPM_OK;
+}
+
+void source_from(const dpm_source* src, SourceParams& S) {
+  std::memset(&S, 0, sizeof(S));
+  S.kind = src->kind;
+  S.particle = src->particle;
+  S.energy = src->energy;
+  S.emin = src->emin;
+  S.emax = src->emax;
+  S.x0 = src->x0;
+  S.y0 = src->y0;
+  S.fx = src->field_x;
+  S.fy = src->field_y;
+}
+
+int check_source(const dpm_tables* t, int particle, double e, double emin, double emax) {
+  if (particle != DPM_ELECTRON && particle != DPM_PHOTON && particle != DPM_POSITRON)
+    return set_error(DPM_E_INVALID, "unknown primary particle");
+  const double lo = emax > emin ? emin : e;
+  const double hi = emax > emin ? emax : e;
+  const double cut = particle == DPM_PHOTON ? t->gcut : t->ecut;
+  if (!(lo > cut) || !(hi < 21.0) || !std::isfinite(hi))
+    return set_error(DPM_E_INVALID, "primary energy must lie above the tracking cut and below the table maximum");
+  return DPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dpm_last_error(void) { return g_last_error.c_str(); }
+int dpm_abi_version(void) { return DPM_ABI_VERSION; }
+
+int dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out) {
+  if (!out) return set_error(DPM_E_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc != DPM_OK) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(DPM_E_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_error(DPM_E_INVALID, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  dpm_tables* t = new dpm_tables();
+  t->device = device;
+  t->nmat = desc->nmat;
+  t->ecut = desc->electron_cut;
+  t->gcut = desc->gamma_cut;
+  t->rho.assign(desc->density, desc->density + desc->nmat);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete t; return set_error(DPM_E_DEVICE, "device query failed"); }
+  t->num_cu = prop.multiProcessorCount;
+  if ((rc = build_pack<float>(desc, t->tf, &t->dev_f)) != DPM_OK || (rc = build_pack<double>(desc, t->td, &t->dev_d)) != DPM_OK) {
+    dpm_tables_destroy(t);
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&t->ctl, 16 * sizeof(int)) != hipSuccess || hipMalloc(&t->stats, sizeof(StatsDev)) != hipSuccess) {
+    dpm_tables_destroy(t);
+    return set_error(DPM_E_DEVICE, "cannot create the run stream or control block");
+  }
+  for (auto& e : t->ev)
+    if (hipEventCreate(&e) != hipSuccess) { dpm_tables_destroy(t); return set_error(DPM_E_DEVICE, "event creation failed"); }
+  *out = t;
+  return DPM_OK;
+}
+
+int dpm_tables_load(const char* dir, int device, dpm_tables** out) {
+  if (!dir || !out) return set_error(DPM_E_INVALID, "null argument");
+  SimMaterialData mat;
+  SimElectronData el;
+  SimPhotonData ph;
+  std::string err;
+  if (!mat.Load(dir, err) || !el.Load(dir, err) || !ph.Load(dir, err)) return set_error(DPM_E_IO, err);
+  if ((int)el.fITr1MFP.size() != mat.fNumMaterial || (int)el.fDEDX.size() != mat.fNumMaterial ||
+      (int)el.fIMFPBrem.size() != mat.fNumMaterial || (int)ph.fIMFPTotal.size() != mat.fNumMaterial ||
+      el.fSBTables.nmat != mat.fNumMaterial)
+    return set_error(DPM_E_IO, "table files disagree on the number of materials");
+  SimTableView view;
+  FlattenTables(mat, el, ph, view);
+  return dpm_tables_create(&view.desc, device, out);
+}
+
+void dpm_tables_destroy(dpm_tables* t) {
+  if (!t) return;
+  (void)hipSetDevice(t->device);
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  t->wsf.release();
+  t->wsd.release();
+  if (t->csr_col) (void)hipFree(t->csr_col);
+  if (t->csr_val) (void)hipFree(t->csr_val);
+  if (t->dev_f) (void)hipFree(t->dev_f);
+  if (t->dev_d) (void)hipFree(t->dev_d);
+  if (t->ctl) (void)hipFree(t->ctl);
+  if (t->stats) (void)hipFree(t->stats);
+  for (auto& e : t->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+int dpm_tables_num_materials(const dpm_tables* t) { return t ? t->nmat : 0; }
+
+int dpm_phantom_create_slab(const dpm_tables* t, double lbox, int geom_index, dpm_phantom** out) {
+  if (!t || !out) return set_error(DPM_E_INVALID, "null argument");
+  if (!(lbox >= 0.05) || !(lbox <= 200.0)) return set_error(DPM_E_INVALID, "voxel size must be in [0.05, 200] mm");
+  if (geom_index < 0) return set_error(DPM_E_INVALID, "negative geometry index");
+  const int need = geom_index == 1 || geom_index == 2 ? 2 : (geom_index == 3 ? 3 : 1);
+  if (t->nmat < need) return set_error(DPM_E_INVALID, "the slab geometry needs more materials than the tables hold");
+  dpm_phantom* p = new dpm_phantom();
+  p->owner = t;
+  p->geom = GEOM_SLAB;
+  p->lbox = lbox;
+  p->geom_index = geom_index;
+  p->nbins = (int)std::floor(100.0 / lbox + 0.5) + 2;
+  p->nvox = p->nbins;
+  *out = p;
+  return DPM_OK;
+}
+
+int dpm_phantom_create_voxel(const dpm_tables* t, int nx, int ny, int nz, double lbox, const uint8_t* mat,
+                             const float* rho, dpm_phantom** out) {
+  if (!t || !out || !mat || !rho) return set_error(DPM_E_INVALID, "null argument");
+  if (nx < 1 || ny < 1 || nz < 1 || (int64_t)nx * ny * nz > ((int64_t)1 << 31))
+    return set_error(DPM_E_INVALID, "phantom dimensions out of range");
+  if (!(lbox > 0.0)) return set_error(DPM_E_INVALID, "voxel size must be positive");
+  const int64_t nv = (int64_t)nx * ny * nz;
+  double scale = 1.0;
+  for (int64_t k = 0; k < nv; ++k) {
+    if (mat[k] >= t->nmat) return set_error(DPM_E_INVALID, "voxel material index out of range");
+    if (!(rho[k] > 0.0f) || !std::isfinite(rho[k])) return set_error(DPM_E_INVALID, "voxel density must be positive");
+    scale = std::max(scale, (double)rho[k] / t->rho[mat[k]]);
+  }
+  HIP_TRY(hipSetDevice(t->device));
+  dpm_phantom* p = new dpm_phantom();
+  p->owner = t;
+  p->geom = GEOM_VOXEL;
+  p->lbox = lbox;
+  p->nx = nx; p->ny = ny; p->nz = nz;
+  p->nvox = nv;
+  p->maj_scale = scale;
+  if (hipMalloc(&p->d_mat, nv) != hipSuccess || hipMalloc(&p->d_rho, nv * sizeof(float)) != hipSuccess ||
+      hipMemcpy(p->d_mat, mat, nv, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->d_rho, rho, nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    dpm_phantom_destroy(p);
+    return set_error(DPM_E_NOMEM, "cannot upload the phantom");
+  }
+  *out = p;
+  return DPM_OK;
+}
+
+void dpm_phantom_destroy(dpm_phantom* p) {
+  if (!p) return;
+  if (p->d_mat) (void)hipFree(p->d_mat);
+  if (p->d_rho) (void)hipFree(p->d_rho);
+  delete p;
+}
+
+int64_t dpm_phantom_num_bins(const dpm_phantom* p) { return p ? (p->geom == GEOM_SLAB ? p->nbins : p->nvox) : 0; }
+
+int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
+                 const dpm_run_options* opt_in, double* tally, double* steps, dpm_stats* st) {
+  if (!t || !p || !src || !tally) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (nhist == 0) return set_error(DPM_E_INVALID, "zero histories");
+  if (nhist > 0xFFFFFFFFull) return set_error(DPM_E_INVALID, "at most 2^32-1 histories per call");
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  if (opt_in) opt = *opt_in;
+  if (opt.hist_begin + nhist > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (src->kind != DPM_SRC_PENCIL && src->kind != DPM_SRC_FIELD) return set_error(DPM_E_INVALID, "unknown source kind");
+  if (opt.nbatch > 1 && (uint64_t)opt.nbatch > nhist) return set_error(DPM_E_INVALID, "more batches than histories");
+  int rc = check_source(t, src->particle, src->energy, src->emin, src->emax);
+  if (rc != DPM_OK) return rc;
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  SourceParams S;
+  source_from(src, S);
+
+  const bool slab = p->geom == GEOM_SLAB;
+  const int nbatch = (!slab && opt.nbatch > 1) ? opt.nbatch : 0;
+  const int64_t nb = slab ? p->nbins : p->nvox;
+  // tally rows: one, or the batches of one chunk
+  int64_t chunk = opt.chunk > 0 ? opt.chunk : (int64_t)1 << 20;
+  int rows = 1;
+  if (nbatch > 0) {
+    const uint64_t per = nhist / nbatch + 1;
+    rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)chunk / per));
+  }
+  double *d_rows = nullptr, *d_steps = nullptr, *d_sum = nullptr, *d_sum2 = nullptr, *d_he = nullptr;
+  auto cleanup = [&]() {
+    if (d_rows) (void)hipFree(d_rows);
+    if (d_steps) (void)hipFree(d_steps);
+    if (d_sum) (void)hipFree(d_sum);
+    if (d_sum2) (void)hipFree(d_sum2);
+    if (d_he) (void)hipFree(d_he);
+  };
+  if (hipMalloc(&d_rows, sizeof(double) * nb * rows) != hipSuccess ||
+      (slab && hipMalloc(&d_steps, sizeof(double) * nb) != hipSuccess) ||
+      (nbatch > 0 && (hipMalloc(&d_sum, sizeof(double) * nb) != hipSuccess ||
+                      hipMalloc(&d_sum2, sizeof(double) * nb) != hipSuccess)) ||
+      (opt.hist_edep && hipMalloc(&d_he, sizeof(double) * nhist) != hipSuccess)) {
+    cleanup();
+    return set_error(DPM_E_NOMEM, "cannot allocate the tally");
+  }
+  (void)hipMemsetAsync(d_rows, 0, sizeof(double) * nb * rows, s);
+  if (d_steps) (void)hipMemsetAsync(d_steps, 0, sizeof(double) * nb, s);
+  if (d_sum) (void)hipMemsetAsync(d_sum, 0, sizeof(double) * nb, s);
+  if (d_sum2) (void)hipMemsetAsync(d_sum2, 0, sizeof(double) * nb, s);
+  if (d_he) (void)hipMemsetAsync(d_he, 0, sizeof(double) * nhist, s);
+  (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), s);
+  dpm_stats local;
+  std::memset(&local, 0, sizeof(local));
+  rc = opt.precision == DPM_FP64
+           ? run_transport<double>(t, p, S, nhist, &opt, t->wsd, t->td, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local)
+           : run_transport<float>(t, p, S, nhist, &opt, t->wsf, t->tf, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local);
+  if (rc == DPM_OK) rc = finish_stats(t, &local);
+  if (rc == DPM_OK) {
+    const double* res = nbatch > 0 ? d_sum : d_rows;
+    if (hipMemcpy(tally, res, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally copy failed");
+    if (rc == DPM_OK && steps) {
+      const double* s2 = slab ? d_steps : d_sum2;
+      if (s2 && hipMemcpy(steps, s2, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(DPM_E_DEVICE, "tally copy failed");
+      if (!s2) std::memset(steps, 0, sizeof(double) * nb);
+    }
+    if (rc == DPM_OK && opt.hist_edep &&
+        hipMemcpy(opt.hist_edep, d_he, sizeof(double) * nhist, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = set_error(DPM_E_DEVICE, "per-history copy failed");
+    if (rc == DPM_OK) {
+      double sc = 0;
+      std::vector<double> h(nb);
+      (void)hipMemcpy(h.data(), res, sizeof(double) * nb, hipMemcpyDeviceToHost);
+      for (double v : h) sc += v;
+      local.e_scored = sc;
+    }
+  }
+  cleanup();
+  if (st) *st = local;
+  return rc;
+}
+
+int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb,
+                          const dpm_run_options* opt_in, double rel_threshold, int64_t* row_ptr, dpm_stats* st) {
+  if (!t || !p || !b || nb <= 0 || !row_ptr) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (p->geom != GEOM_VOXEL) return set_error(DPM_E_INVALID, "beamlets need a voxel phantom");
+  if (nb > (1 << 20)) return set_error(DPM_E_INVALID, "too many beamlets");
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  if (opt_in) opt = *opt_in;
+  uint64_t total = 0;
+  std::vector<uint64_t> off(nb);
+  std::vector<double> bx(nb), by(nb), be(nb);
+  std::vector<int> bt(nb);
+  for (int i = 0; i < nb; ++i) {
+    int rc = check_source(t, b[i].particle, b[i].energy, 0, 0);
+    if (rc != DPM_OK) return rc;
+    if (b[i].nhist == 0) return set_error(DPM_E_INVALID, "beamlet without histories");
+    off[i] = opt.hist_begin + total;
+    total += b[i].nhist;
+    bx[i] = b[i].x0; by[i] = b[i].y0; be[i] = b[i].energy; bt[i] = b[i].particle;
+  }
+  if (opt.hist_begin + total > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
+  double* d_rows = nullptr;
+  void* d_bl = nullptr;
+  double* d_rmax = nullptr;
+  int64_t *d_cnt = nullptr, *d_rowptr = nullptr;
+  if (t->csr_col) (void)hipFree(t->csr_col);
+  if (t->csr_val) (void)hipFree(t->csr_val);
+  t->csr_col = nullptr;
+  t->csr_val = nullptr;
+  t->csr_nnz = 0;
+  auto cleanup = [&]() {
+    for (void* q : {(void*)d_rows, d_bl, (void*)d_rmax, (void*)d_cnt, (void*)d_rowptr})
+      if (q) (void)hipFree(q);
+  };
+  const size_t blbytes = (size_t)nb * (sizeof(uint64_t) + 3 * sizeof(double) + sizeof(int)) + 1024;
+  if (hipMalloc(&d_rows, rows_bytes) != hipSuccess || hipMalloc(&d_bl, blbytes) != hipSuccess ||
+      hipMalloc(&d_rmax, sizeof(double) * nb) != hipSuccess || hipMalloc(&d_cnt, sizeof(int64_t) * nb) != hipSuccess ||
+      hipMalloc(&d_rowptr, sizeof(int64_t) * (nb + 1)) != hipSuccess) {
+    cleanup();
+    return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string(rows_bytes >> 20) + " MiB)");
+  }
+  unsigned char* q = (unsigned char*)d_bl;
+  uint64_t* d_off = (uint64_t*)q; q += ((sizeof(uint64_t) * nb + 255) & ~(size_t)255);
+  double* d_bx = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
+  double* d_by = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
+  double* d_be = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
+  int* d_bt = (int*)q;
+  (void)hipMemcpy(d_off, off.data(), sizeof(uint64_t) * nb, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_bx, bx.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_by, by.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_be, be.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_bt, bt.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+  (void)hipMemsetAsync(d_rows, 0, rows_bytes, s);
+  (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), s);
+  SourceParams S;
+  std::memset(&S, 0, sizeof(S));
+  S.kind = DPM_SRC_PENCIL;
+  S.nbeamlet = nb;
+  S.bl_offset = d_off; S.bl_x = d_bx; S.bl_y = d_by; S.bl_e = d_be; S.bl_type = d_bt;
+  dpm_stats local;
+  std::memset(&local, 0, sizeof(local));
+  opt.nbatch = 0;
+  int rc = opt.precision == DPM_FP64
+               ? run_transport<double>(t, p, S, total, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
+               : run_transport<float>(t, p, S, total, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
+  if (rc == DPM_OK) rc = finish_stats(t, &local);
+  if (rc == DPM_OK) {
+    hipLaunchKernelGGL(k_rowmax, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax);
+    hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr, d_cnt,
+                       (int32_t*)nullptr, (double*)nullptr, 0);
+    std::vector<int64_t> cnt(nb);
+    if (hipMemcpy(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = set_error(DPM_E_DEVICE, "CSR count failed");
+    if (rc == DPM_OK) {
+      row_ptr[0] = 0;
+      for (int i = 0; i < nb; ++i) row_ptr[i + 1] = row_ptr[i] + cnt[i];
+      const int64_t nnz = row_ptr[nb];
+      if (nnz > 0) {
+        if (hipMalloc(&t->csr_col, sizeof(int32_t) * nnz) != hipSuccess ||
+            hipMalloc(&t->csr_val, sizeof(double) * nnz) != hipSuccess) {
+          rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
+        } else {
+          (void)hipMemcpy(d_rowptr, row_ptr, sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice);
+          hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr,
+                             d_cnt, t->csr_col, t->csr_val, 1);
+          if (hipStreamSynchronize(s) != hipSuccess) rc = set_error(DPM_E_DEVICE, "CSR build failed");
+          else t->csr_nnz = nnz;
+        }
+      }
+    }
+  }
+  cleanup();
+  if (st) *st = local;
+  return rc;
+}
+
+int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val) {
+  if (!t || !col || !val) return set_error(DPM_E_INVALID, "null argument");
+  if (t->csr_nnz == 0) return DPM_OK;
+  HIP_TRY(hipSetDevice(t->device));
+  HIP_TRY(hipMemcpy(col, t->csr_col, sizeof(int32_t) * t->csr_nnz, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(val, t->csr_val, sizeof(double) * t->csr_nnz, hipMemcpyDeviceToHost));
+  return DPM_OK;
+}
+
+int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* energy, const int* imat,
+               const uint32_t* r1, const uint32_t* r2, const uint32_t* r3, double* out, int32_t* ebin, int32_t* abin) {
+  if (!t || !energy || !r1 || !r2 || !out || !ebin || !abin || n < 0) return set_error(DPM_E_INVALID, "null argument");
+  if (kind < 0 || kind > 3) return set_error(DPM_E_INVALID, "unknown sampler");
+  if (kind <= 2 && !r3) return set_error(DPM_E_INVALID, "sampler needs three uniforms");
+  if (kind == 0 && !imat) return set_error(DPM_E_INVALID, "SB sampler needs material indices");
+  if (n == 0) return DPM_OK;
+  for (int i = 0; i < n; ++i) {
+    if ((r1[i] >> 24) || (r2[i] >> 24) || (r3 && (r3[i] >> 24))) return set_error(DPM_E_INVALID, "uniforms must be 24-bit");
+    if (kind == 0 && (imat[i] < 0 || imat[i] >= t->nmat)) return set_error(DPM_E_INVALID, "material index out of range");
+    if (!(energy[i] > 0.0) || !(energy[i] < 21.0)) return set_error(DPM_E_INVALID, "energy outside the tables");
+  }
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  void* buf = nullptr;
+  const size_t bytes = (size_t)n * (8 + 4 + 4 * 3 + 8 + 4 + 4) + 8 * 256;
+  HIP_TRY(hipMalloc(&buf, bytes));
+  unsigned char* q = (unsigned char*)buf;
+  auto take = [&](size_t nbytes) { unsigned char* r = q; q += (nbytes + 255) & ~(size_t)255; return r; };
+  double* de = (double*)take(8 * (size_t)n);
+  int* dm = (int*)take(4 * (size_t)n);
+  uint32_t* d1 = (uint32_t*)take(4 * (size_t)n);
+  uint32_t* d2 = (uint32_t*)take(4 * (size_t)n);
+  uint32_t* d3 = (uint32_t*)take(4 * (size_t)n);
+  double* dout = (double*)take(8 * (size_t)n);
+  int* deb = (int*)take(4 * (size_t)n);
+  int* dab = (int*)take(4 * (size_t)n);
+  int rc = DPM_OK;
+  hipError_t e = hipMemcpy(de, energy, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(dm, 0, 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMemcpy(d1, r1, 4 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d2, r2, 4 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = r3 ? hipMemcpy(d3, r3, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(d3, 0, 4 * (size_t)n);
+  if (e == hipSuccess)
+    e = precision == DPM_FP64 ? launch_sample_f64(t->td, kind, n, de, dm, d1, d2, d3, dout, deb, dab, s)
+                              : launch_sample_f32(t->tf, kind, n, de, dm, d1, d2, d3, dout, deb, dab, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ebin, deb, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(abin, dab, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = set_error(DPM_E_DEVICE, std::string("dpm_sample: ") + hipGetErrorString(e));
+  (void)hipFree(buf);
+  return rc;
+}
+
+int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energy, const int* imat, double* out) {
+  if (!t || !energy || !out || n < 0) return set_error(DPM_E_INVALID, "null argument");
+  if (which < 0 || which > 8) return set_error(DPM_E_INVALID, "unknown table");
+  if (n == 0) return DPM_OK;
+  for (int i = 0; i < n; ++i) {
+    if (imat && (imat[i] < 0 || imat[i] >= t->nmat)) return set_error(DPM_E_INVALID, "material index out of range");
+    if (!(energy[i] > 0.0) || !std::isfinite(energy[i])) return set_error(DPM_E_INVALID, "energy must be positive");
+  }
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  void* buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, (size_t)n * 20 + 1024));
+  double* de = (double*)buf;
+  int* dm = (int*)((unsigned char*)buf + (((size_t)n * 8 + 255) & ~(size_t)255));
+  double* dout = (double*)((unsigned char*)dm + (((size_t)n * 4 + 255) & ~(size_t)255));
+  int rc = DPM_OK;
+  hipError_t e = hipMemcpy(de, energy, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(dm, 0, 4 * (size_t)n);
+  if (e == hipSuccess)
+    e = precision == DPM_FP64 ? launch_eval_f64(t->td, which, n, de, dm, dout, s) : launch_eval_f32(t->tf, which, n, de, dm, dout, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = set_error(DPM_E_DEVICE, std::string("dpm_eval: ") + hipGetErrorString(e));
+  (void)hipFree(buf);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,687 @@
+// dpm_kernels.hip -- event-based transport kernels of the MI355X DPM engine.
+//
+// Compiled twice: with DPM_REAL=float (production FP32 transport, FP64 tally)
+// and DPM_REAL=double (FP64-everywhere check path, no FMA contraction so its
+// arithmetic follows the CPU restatement operation for operation).
+//
+// Work decomposition (replaces the history loop and LIFO TrackStack of
+// Simulate, SimDPMLike.cc:57-247, TrackStack.hh:17-59):
+//   - particles live in SoA queues (one for photons, one for e-/e+);
+//   - k_source fills the queues with the primaries of a chunk of histories;
+//   - k_photons / k_electrons are persistent kernels: each wave keeps 64
+//     particles in flight and refills finished lanes from the input queue with
+//     one wave-aggregated atomic; secondaries are appended to the output
+//     queues with ballot + prefix-count compaction (one atomic per wave);
+//   - the host alternates photon/electron rounds until both queues drain.
+// Every particle carries its own Philox stream (seed, history, serial), so
+// results do not depend on the processing order.
+#include "dpm_device.hpp"
+#include "dpm_engine.hpp"
+
+#ifndef DPM_REAL
+#error "DPM_REAL must be float or double"
+#endif
+
+namespace dpm {
+
+using R = DPM_REAL;
+
+constexpr int kBlock = 256;
+constexpr int kSlabLds = 1024;         // depth bins privatised in LDS
+constexpr uint32_t kMaxIter = 200000;  // per-particle iteration cap (termination guard)
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// reserve one slot per active lane of the (possibly divergent) wave with a
+// single atomic; returns this lane's slot
+__device__ __forceinline__ int wave_reserve(int* counter) {
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll((long long)act) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, __popcll(act));
+  base = __shfl(base, leader);
+  return base + (int)__popcll(act & lanes_below());
+}
+
+// ---------------------------------------------------------------------------
+// per-block accumulators (LDS), flushed once per block
+// ---------------------------------------------------------------------------
+template <int NS>
+struct BlockAccT {
+  double escaped;
+  unsigned cnt[kNumCounters];
+  int maxiz;
+  double slab_e[NS];
+  unsigned slab_n[NS];
+};
+template <int GEOM> struct AccOf { using type = BlockAccT<(GEOM == GEOM_SLAB) ? kSlabLds : 1>; };
+
+template <class BlockAcc>
+__device__ __forceinline__ void acc_init(BlockAcc& a, bool slab) {
+  for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x) a.cnt[i] = 0u;
+  if (threadIdx.x == 0) { a.escaped = 0.0; a.maxiz = -1; }
+  if (slab)
+    for (int i = threadIdx.x; i < (int)(sizeof(a.slab_n) / sizeof(unsigned)); i += blockDim.x) {
+      a.slab_e[i] = 0.0;
+      a.slab_n[i] = 0u;
+    }
+  __syncthreads();
+}
+
+template <class BlockAcc>
+__device__ __forceinline__ void acc_flush(BlockAcc& a, const KParams<R>& P, bool slab) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x)
+    if (a.cnt[i]) atomicAdd((unsigned long long*)&P.stats->cnt[P.kind][i], (unsigned long long)a.cnt[i]);
+  if (threadIdx.x == 0) {
+    if (a.escaped != 0.0) unsafeAtomicAdd(&P.stats->escaped, a.escaped);
+    if (a.maxiz >= 0) atomicMax(&P.stats->maxiz, a.maxiz);
+  }
+  if (slab) {
+    const int ns = (int)(sizeof(a.slab_n) / sizeof(unsigned));
+    const int nb = P.nbins < ns ? P.nbins : ns;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      if (a.slab_n[i]) {
+        unsafeAtomicAdd(&P.tally[i], a.slab_e[i]);
+        unsafeAtomicAdd(&P.steps[i], (double)a.slab_n[i]);
+      }
+    }
+  }
+}
+
+template <class BlockAcc>
+__device__ __forceinline__ void count(BlockAcc& a, int which) { atomicAdd(&a.cnt[which], 1u); }
+
+// Geom::Score (Geom.cc:120-130) for the slab; voxel tally row `tag`
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ void score(BlockAcc& a, const KParams<R>& P, R e, const int box[3], uint32_t tag,
+                                      uint32_t hist) {
+  count(a, kCntDeposits);
+  if (GEOM == GEOM_SLAB) {
+    const int iz = box[2];
+    if (iz < 0) return;
+    atomicMax(&a.maxiz, iz);
+    if (iz < (int)(sizeof(a.slab_n) / sizeof(unsigned)) && iz < kSlabLds) {
+      atomicAdd(&a.slab_e[iz], (double)e);
+      atomicAdd(&a.slab_n[iz], 1u);
+    } else if (iz < P.nbins) {
+      unsafeAtomicAdd(&P.tally[iz], (double)e);
+      unsafeAtomicAdd(&P.steps[iz], 1.0);
+    }
+  } else {
+    const size_t k = ((size_t)box[2] * P.G.ny + box[1]) * P.G.nx + box[0];
+    unsafeAtomicAdd(&P.tally[(size_t)tag * P.nvox + k], (double)e);
+  }
+  if (P.hist_edep) unsafeAtomicAdd(&P.hist_edep[hist - P.hist_base], (double)e);
+}
+
+template <class BlockAcc>
+__device__ __forceinline__ void escape(BlockAcc& a, R e) {
+  if (e != (R)0) atomicAdd(&a.escaped, (double)e);
+}
+
+// ---------------------------------------------------------------------------
+// queue records
+// ---------------------------------------------------------------------------
+// type code: 0 photon, 1 electron, 2 positron; tt = type | ndraw << 2 | tag << 8
+__device__ __forceinline__ uint32_t make_tt(int type, uint32_t ndraw, uint32_t tag) {
+  return (uint32_t)type | (ndraw << 2) | (tag << 8);
+}
+
+__device__ __forceinline__ void push(const QueueView<R>& q, int* overflow, const R pos[3], R u, R v, R w, R e,
+                                     uint32_t hist, uint64_t serial, int type, uint32_t tag) {
+  const int slot = wave_reserve(q.count);
+  if (slot >= q.cap) { atomicAdd(overflow, 1); return; }
+  q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
+  q.u[slot] = u; q.v[slot] = v; q.w[slot] = w; q.e[slot] = e;
+  q.hist[slot] = hist;
+  q.slo[slot] = (uint32_t)serial;
+  q.shi[slot] = (uint32_t)(serial >> 32);
+  q.tt[slot] = make_tt(type, 0u, tag);
+}
+
+struct Track {
+  R pos[3];
+  R dir[3];
+  R E;
+  int box[3];
+  int type;        // 0 photon, 1 e-, 2 e+
+  uint32_t tag;
+  uint32_t iter;
+  Rng rng;
+};
+
+__device__ __forceinline__ void load_track(const QueueView<R>& q, int i, const KParams<R>& P, Track& t) {
+  t.pos[0] = q.x[i]; t.pos[1] = q.y[i]; t.pos[2] = q.z[i];
+  t.dir[0] = q.u[i]; t.dir[1] = q.v[i]; t.dir[2] = q.w[i];
+  t.E = q.e[i];
+  const uint32_t tt = q.tt[i];
+  t.type = (int)(tt & 3u);
+  t.tag = tt >> 8;
+  t.iter = 0;
+  t.rng.k0 = P.k0; t.rng.k1 = P.k1;
+  t.rng.hist = q.hist[i];
+  t.rng.slo = q.slo[i];
+  t.rng.shi = q.shi[i];
+  t.rng.ndraw = (tt >> 2) & 63u;
+  t.rng.nchild = 0;
+}
+
+// PerformAnnihilation (SimDPMLike.cc:753-791): two back-to-back 0.511 photons
+template <class BlockAcc>
+__device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Track& t, const QueueView<R>& qg) {
+  count(a, kCntAnnih);
+  const R cost = (R)1.0 - (R)2.0 * t.rng.uni<R>();
+  const R sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
+  const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
+  R sp, cp;
+  rsincos(phi, &sp, &cp);
+  const R rx = sint * cp, ry = sint * sp, rz = cost;
+  count(a, kCntSecondaries);
+  push(qg, P.overflow, t.pos, rx, ry, rz, (R)kEMC2, t.rng.hist, t.rng.next_child(), 0, t.tag);
+  count(a, kCntSecondaries);
+  push(qg, P.overflow, t.pos, -rx, -ry, -rz, (R)kEMC2, t.rng.hist, t.rng.next_child(), 0, t.tag);
+}
+
+// ---------------------------------------------------------------------------
+// source: primaries of histories [h0, h0+n) (SimDPMLike.cc:64-75, extended)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S, uint64_t h0, int n,
+                                                   QueueView<R> qe, QueueView<R> qg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t hid = h0 + (uint64_t)i;
+  Rng rng;
+  rng.k0 = P.k0; rng.k1 = P.k1;
+  rng.hist = (uint32_t)hid;
+  rng.slo = 0; rng.shi = 0; rng.ndraw = 0; rng.nchild = 0;
+  int bl = 0;
+  double x0 = S.x0, y0 = S.y0, e = S.energy;
+  int particle = S.particle;
+  if (S.nbeamlet > 0) {
+    // beamlet of this history: last offset <= hid (offsets are ascending)
+    int lo = 0, hi = S.nbeamlet - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (S.bl_offset[mid] <= hid) lo = mid; else hi = mid - 1;
+    }
+    bl = lo;
+    x0 = S.bl_x[bl]; y0 = S.bl_y[bl]; e = S.bl_e[bl]; particle = S.bl_type[bl];
+  }
+  R pos[3] = {(R)x0, (R)y0, (R)S.z0};
+  if (S.kind == 1) {
+    const R ux = rng.uni<R>(), uy = rng.uni<R>();
+    pos[0] = (R)x0 + (ux - (R)0.5) * (R)S.fx;
+    pos[1] = (R)y0 + (uy - (R)0.5) * (R)S.fy;
+  }
+  R E = (R)e;
+  if (S.emax > S.emin) E = (R)S.emin + rng.uni<R>() * (R)(S.emax - S.emin);
+  uint32_t tag = 0;
+  if (S.nbeamlet > 0) tag = (uint32_t)bl;
+  else if (S.nbatch > 1) {
+    const uint64_t rel = hid - S.hist_begin;
+    tag = (uint32_t)((rel * (uint64_t)S.nbatch) / S.nhist_total) - S.batch_first;
+  }
+  const QueueView<R>& q = particle == 0 ? qg : qe;
+  const int type = particle == 0 ? 0 : (particle < 0 ? 1 : 2);
+  const int slot = atomicAdd(q.count, 1);
+  if (slot >= q.cap) { atomicAdd(P.overflow, 1); return; }
+  q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
+  q.u[slot] = (R)0.0; q.v[slot] = (R)0.0; q.w[slot] = (R)1.0;
+  q.e[slot] = E;
+  q.hist[slot] = (uint32_t)hid;
+  q.slo[slot] = 0; q.shi[slot] = 0;
+  q.tt[slot] = make_tt(type, rng.ndraw, tag);
+}
+
+// ---------------------------------------------------------------------------
+// photons: KeepTrackingPhoton (SimDPMLike.cc:463-635), one Woodcock step per
+// loop trip; e-/e+ go to qe (this round's electron queue), annihilation
+// photons to qg (next round's photon queue)
+// ---------------------------------------------------------------------------
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Track& t, const QueueView<R>& qe,
+                                            const QueueView<R>& qg) {
+  const Tables<R>& T = P.T;
+  const R kInvEMC2 = (R)(1.0 / kEMC2);
+  count(a, kCntGSteps);
+  const R globalMaxMFP = (R)1.0 / (linear_value(T.gmax, t.E, 0) * P.majScale);
+  const R stepLength = -globalMaxMFP * t.rng.lnu<R>();
+  t.pos[0] += t.dir[0] * stepLength;
+  t.pos[1] += t.dir[1] * stepLength;
+  t.pos[2] += t.dir[2] * stepLength;
+  if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) { escape(a, t.E); return false; }
+  count(a, kCntVoxelVisits);
+  const int imat = material_index<R, GEOM>(P.G, t.box);
+  if (imat < 0) { escape(a, t.E); return false; }
+  const R rho = voxel_density<R, GEOM>(P.G, imat, t.box);
+  const R totalIMFP = linear_value(T.ptot, t.E, imat) * rho;
+  const R r1 = t.rng.uni<R>();
+  R theProb = (R)1.0 - totalIMFP * globalMaxMFP;
+  if (r1 < theProb) return true;
+  theProb += linear_value(T.pcomp, t.E, imat) * rho * globalMaxMFP;
+  if (r1 < theProb) {
+    // Compton (SimDPMLike.cc:512-570)
+    count(a, kCntKN);
+    const uint32_t q1 = t.rng.draw24(), q2 = t.rng.draw24(), q3 = t.rng.draw24();
+    const R theEps = sample_kn(T.kn, t.E, q1, q2, q3, (int*)nullptr, (int*)nullptr);
+    const R kappa = t.E * kInvEMC2;
+    const R phCost = (R)1.0 - ((R)1.0 - theEps) / (theEps * kappa);
+    const R phEner = theEps * t.E;
+    const R phPhi = (R)(2.0 * kPI) * t.rng.uni<R>();
+    const R elEner = t.E - phEner;
+    if (elEner < T.ecut) {
+      score<GEOM>(a, P, elEner, t.box, t.tag, t.rng.hist);
+    } else {
+      const R e0 = t.E * kInvEMC2;
+      const R elCost = ((R)1.0 + e0) * rsqrt_(((R)1.0 - theEps) / (e0 * ((R)2.0 + e0 * ((R)1.0 - theEps))));
+      const R sint = rsqrt_(((R)1.0 + elCost) * ((R)1.0 - elCost));
+      const R phi = phPhi + (R)kPI;
+      R sp, cp;
+      rsincos(phi, &sp, &cp);
+      R du = sint * cp, dv = sint * sp, dw = elCost;
+      rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
+      count(a, kCntSecondaries);
+      push(qe, P.overflow, t.pos, du, dv, dw, elEner, t.rng.hist, t.rng.next_child(), 1, t.tag);
+    }
+    t.E = phEner;
+    if (t.E < T.gcut) {
+      score<GEOM>(a, P, t.E, t.box, t.tag, t.rng.hist);
+      return false;
+    }
+    const R phSint = rsqrt_(((R)1.0 - phCost) * ((R)1.0 + phCost));
+    R sp, cp;
+    rsincos(phPhi, &sp, &cp);
+    R u1 = phSint * cp, u2 = phSint * sp, u3 = phCost;
+    rotate_to_lab(u1, u2, u3, t.dir[0], t.dir[1], t.dir[2]);
+    t.dir[0] = u1; t.dir[1] = u2; t.dir[2] = u3;
+    return true;
+  }
+  theProb += linear_value(T.ppair, t.E, imat) * rho * globalMaxMFP;
+  if (r1 < theProb) {
+    // pair production (SimDPMLike.cc:573-627)
+    count(a, kCntPair);
+    const R sumEkin = t.E - (R)(2.0 * kEMC2);
+    const R e1 = t.rng.uni<R>() * sumEkin;
+    const R e2 = sumEkin - e1;
+    if (e2 < T.ecut) {
+      score<GEOM>(a, P, e2, t.box, t.tag, t.rng.hist);
+    } else {
+      count(a, kCntSecondaries);
+      push(qe, P.overflow, t.pos, t.dir[0], t.dir[1], t.dir[2], e2, t.rng.hist, t.rng.next_child(), 1, t.tag);
+    }
+    if (e1 < T.ecut) {
+      score<GEOM>(a, P, e1, t.box, t.tag, t.rng.hist);
+      annihilate(a, P, t, qg);
+    } else {
+      count(a, kCntSecondaries);
+      push(qe, P.overflow, t.pos, t.dir[0], t.dir[1], t.dir[2], e1, t.rng.hist, t.rng.next_child(), 2, t.tag);
+    }
+    return false;
+  }
+  // photoelectric absorption (SimDPMLike.cc:629-633)
+  score<GEOM>(a, P, t.E, t.box, t.tag, t.rng.hist);
+  return false;
+}
+
+// start of a popped track (SimDPMLike.cc:86-95); false if it ends here
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ bool start_track(BlockAcc& a, const KParams<R>& P, Track& t) {
+  if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) box_of<R, GEOM>(P.G, t.pos, t.box);
+  count(a, kCntVoxelVisits);
+  const int imat = material_index<R, GEOM>(P.G, t.box);
+  if (imat < 0) { escape(a, t.E); return false; }
+  return true;
+}
+
+template <int GEOM>
+__global__ void __launch_bounds__(kBlock) k_photons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
+                                                    QueueView<R> qg) {
+  __shared__ typename AccOf<GEOM>::type acc;
+  acc_init(acc, GEOM == GEOM_SLAB);
+  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  Track t;
+  bool has = false, exhausted = false;
+  for (;;) {
+    if (!exhausted) {
+      const uint64_t need = __ballot(!has);
+      if (need) {
+        const int leader = __ffsll((long long)need) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(head, __popcll(need));
+        base = __shfl(base, leader);
+        if (base + __popcll(need) >= n_in) exhausted = true;
+        if (!has) {
+          const int idx = base + (int)__popcll(need & lanes_below());
+          if (idx < n_in) {
+            load_track(in, idx, P, t);
+            has = start_track<GEOM>(acc, P, t);
+          }
+        }
+      }
+    }
+    if (!__any(has)) {
+      if (exhausted) break;
+      continue;
+    }
+    if (has) {
+      has = photon_step<GEOM>(acc, P, t, qe, qg);
+      if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
+    }
+  }
+  acc_flush(acc, P, GEOM == GEOM_SLAB);
+}
+
+// ---------------------------------------------------------------------------
+// electrons / positrons: the loop of Simulate (SimDPMLike.cc:113-245) around
+// KeepTrackingElectron (SimDPMLike.cc:254-460), flattened so that one trip of
+// the kernel loop is one condensed-history step plus its interaction
+// ---------------------------------------------------------------------------
+struct EState {
+  R nTr1, nTr10, nMol, invMol, nBrem, E0;
+  bool hinge;
+};
+
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, Track& t, EState& s) {
+  if (!start_track<GEOM>(a, P, t)) return false;
+  s.hinge = true;
+  s.E0 = t.E;
+  s.nTr1 = spline_value(P.T.k1, t.E, 0);
+  s.nTr10 = t.rng.uni<R>() * s.nTr1;
+  s.nTr1 -= s.nTr10;
+  s.nMol = -t.rng.lnu<R>();
+  s.invMol = spline_value(P.T.moller, t.E, 0);
+  s.nBrem = -t.rng.lnu<R>();
+  return true;
+}
+
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, Track& t, EState& s,
+                                              const QueueView<R>& qe, const QueueView<R>& qg) {
+  const Tables<R>& T = P.T;
+  // distance to boundary at the start of every step (SimDPMLike.cc:257, 451)
+  const R stepGeom = distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
+  if (stepGeom < (R)0.0) { escape(a, t.E); return false; }
+  count(a, kCntESteps);
+  count(a, kCntVoxelVisits);
+  int imat = material_index<R, GEOM>(P.G, t.box);
+  if (imat < 0) { escape(a, t.E); return false; }
+  const R ecut = T.ecut;
+  const R scalMolMFP = T.mscale[imat];
+  const R rho = voxel_density<R, GEOM>(P.G, imat, t.box);
+  R stepLength = stepGeom;
+  int what = 0;
+  R theDEDX = spline_value(T.dedx, t.E, imat) * rho;
+  R midStepE = rmax(t.E - (R)0.5 * stepLength * theDEDX, ecut);
+  R delNumTr1MFP = spline_value(T.itr1, midStepE, imat) * rho;
+  const R delNumMollerMFP = s.invMol * scalMolMFP * rho;
+  const R delNumBremMFP = spline_value(T.brem, midStepE, imat) * rho;
+  const R stepBrem = s.nBrem / delNumBremMFP;
+  if (stepBrem < stepLength) { stepLength = stepBrem; what = 1; }
+  const R stepMoller = s.nMol / delNumMollerMFP;
+  if (stepMoller < stepLength) { stepLength = stepMoller; what = 2; }
+  R stepElastic = s.nTr1 / delNumTr1MFP;
+  if (stepElastic < stepLength) {
+    stepElastic = s.nTr1 / (spline_value(T.itr1, t.E, imat) * rho);
+    midStepE = rmax(t.E - (R)0.5 * stepElastic * theDEDX, ecut);
+    delNumTr1MFP = spline_value(T.itr1, midStepE, imat) * rho;
+    stepElastic = rmin(stepLength, s.nTr1 / delNumTr1MFP);
+    stepLength = stepElastic;
+    what = 3;
+  }
+  s.nBrem = (what == 1) ? (R)0 : s.nBrem - stepLength * delNumBremMFP;
+  s.nMol = (what == 2) ? (R)0 : s.nMol - stepLength * delNumMollerMFP;
+  s.nTr1 = (what == 3) ? (R)0 : s.nTr1 - stepLength * delNumTr1MFP;
+  midStepE = rmax(t.E - (R)0.5 * stepLength * theDEDX, ecut);
+  theDEDX = spline_value(T.dedx, midStepE, imat) * rho;
+  const R deltE = stepLength * theDEDX;
+  const R eFinal = t.E - deltE;
+  R edep;
+  if (eFinal < ecut) {
+    edep = t.E;
+    t.E = (R)0.0;
+    if (t.type == 2) {
+      // the stopping e+ annihilates one step ahead (SimDPMLike.cc:406-421)
+      t.pos[0] += t.dir[0] * stepLength;
+      t.pos[1] += t.dir[1] * stepLength;
+      t.pos[2] += t.dir[2] * stepLength;
+      distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
+      count(a, kCntVoxelVisits);
+      imat = material_index<R, GEOM>(P.G, t.box);
+      if (imat < 0) { escape(a, edep); return false; }
+      annihilate(a, P, t, qg);
+    }
+    what = 4;
+  } else {
+    t.E = eFinal;
+    edep = deltE;
+  }
+  t.pos[0] += t.dir[0] * stepLength;
+  t.pos[1] += t.dir[1] * stepLength;
+  t.pos[2] += t.dir[2] * stepLength;
+  score<GEOM>(a, P, edep, t.box, t.tag, t.rng.hist);
+  if (what == 0) return true;
+  if (what == 4) return false;
+  // the interaction switch of Simulate (SimDPMLike.cc:169-244)
+  if (what == 1) {
+    if (t.E > T.gcut) {
+      // PerformBrem (SimDPMLike.cc:661-695)
+      count(a, kCntSB);
+      const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24(), k3 = t.rng.draw24();
+      const R eGamma = sample_sb(T.sb, t.E, imat, k1, k2, k3, (int*)nullptr, (int*)nullptr);
+      const R dum0 = (R)(kEMC2 * 0.7071067812) / (t.E + (R)kEMC2);
+      const R cost = rmax((R)-1.0, rmin((R)1.0, (R)1.0 - dum0 * dum0));
+      const R sint = rsqrt_(((R)1.0 + cost) * ((R)1.0 - cost));
+      const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
+      R sp, cp;
+      rsincos(phi, &sp, &cp);
+      R du = sint * cp, dv = sint * sp, dw = cost;
+      rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
+      count(a, kCntSecondaries);
+      push(qg, P.overflow, t.pos, du, dv, dw, eGamma, t.rng.hist, t.rng.next_child(), 0, t.tag);
+      t.E = t.E - eGamma;
+    }
+    if (t.E < ecut) {
+      score<GEOM>(a, P, t.E, t.box, t.tag, t.rng.hist);
+      t.E = (R)0.0;
+      if (t.type == 2) annihilate(a, P, t, qg);
+      return false;
+    }
+    s.nBrem = -t.rng.lnu<R>();
+    return true;
+  }
+  if (what == 2) {
+    if (t.E > (R)2. * ecut) {
+      // PerformMoller (SimDPMLike.cc:699-728)
+      count(a, kCntMoller);
+      const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24(), k3 = t.rng.draw24();
+      const R secEkin = sample_moller(T.mo, t.E, k1, k2, k3, (int*)nullptr, (int*)nullptr);
+      const R k2EMC2 = (R)(2.0 * kEMC2);
+      const R cost = rsqrt_(secEkin * (t.E + k2EMC2) / (t.E * (secEkin + k2EMC2)));
+      const R secCost = rmin((R)1.0, cost);
+      const R sint = rsqrt_(((R)1.0 + secCost) * ((R)1.0 - secCost));
+      const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
+      R sp, cp;
+      rsincos(phi, &sp, &cp);
+      R du = sint * cp, dv = sint * sp, dw = secCost;
+      rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
+      count(a, kCntSecondaries);
+      push(qe, P.overflow, t.pos, du, dv, dw, secEkin, t.rng.hist, t.rng.next_child(), 1, t.tag);
+      t.E -= secEkin;
+    }
+    s.nMol = -t.rng.lnu<R>();
+    s.invMol = spline_value(T.moller, t.E, 0);
+    return true;
+  }
+  // what == 3: MSC hinge or end of the MSC step
+  if (s.hinge) {
+    // PerformMSCAngularDeflection (SimDPMLike.cc:731-750)
+    count(a, kCntGS);
+    const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24();
+    const R omu = sample_gs_omu(T.gs, s.E0, k1, k2, (int*)nullptr, (int*)nullptr);
+    R cost, sint;
+    if (sizeof(R) == 8) {
+      cost = rmax((R)-1.0, rmin((R)1.0, (R)1. - omu));
+      sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
+    } else {
+      const R o = rmax((R)0.0, rmin((R)2.0, omu));
+      cost = (R)1.0 - o;
+      sint = rsqrt_(o * ((R)2.0 - o));
+    }
+    const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
+    R sp, cp;
+    rsincos(phi, &sp, &cp);
+    R u1 = sint * cp, u2 = sint * sp, u3 = cost;
+    rotate_to_lab(u1, u2, u3, t.dir[0], t.dir[1], t.dir[2]);
+    t.dir[0] = u1; t.dir[1] = u2; t.dir[2] = u3;
+    s.nTr1 = s.nTr10;
+    s.hinge = false;
+  } else {
+    s.E0 = t.E;
+    s.nTr1 = spline_value(T.k1, s.E0, 0);
+    s.nTr10 = t.rng.uni<R>() * s.nTr1;
+    s.nTr1 -= s.nTr10;
+    s.hinge = true;
+  }
+  return true;
+}
+
+template <int GEOM>
+__global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
+                                                      QueueView<R> qg) {
+  __shared__ typename AccOf<GEOM>::type acc;
+  acc_init(acc, GEOM == GEOM_SLAB);
+  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  Track t;
+  EState s;
+  bool has = false, exhausted = false;
+  for (;;) {
+    if (!exhausted) {
+      const uint64_t need = __ballot(!has);
+      if (need) {
+        const int leader = __ffsll((long long)need) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(head, __popcll(need));
+        base = __shfl(base, leader);
+        if (base + __popcll(need) >= n_in) exhausted = true;
+        if (!has) {
+          const int idx = base + (int)__popcll(need & lanes_below());
+          if (idx < n_in) {
+            load_track(in, idx, P, t);
+            has = electron_init<GEOM>(acc, P, t, s);
+          }
+        }
+      }
+    }
+    if (!__any(has)) {
+      if (exhausted) break;
+      continue;
+    }
+    if (has) {
+      has = electron_step<GEOM>(acc, P, t, s, qe, qg);
+      if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
+    }
+  }
+  acc_flush(acc, P, GEOM == GEOM_SLAB);
+}
+
+// ---------------------------------------------------------------------------
+// samplers and interpolations with explicit inputs (parity entry points)
+// ---------------------------------------------------------------------------
+__global__ void k_sample(Tables<R> T, int kind, int n, const double* e, const int* imat, const uint32_t* r1,
+                         const uint32_t* r2, const uint32_t* r3, double* out, int* ebin, int* abin) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R E = (R)e[i];
+  int eb = -1, ab = -1;
+  double v = 0;
+  if (kind == 0) v = (double)sample_sb(T.sb, E, imat[i], r1[i], r2[i], r3[i], &eb, &ab);
+  else if (kind == 1) v = (double)sample_moller(T.mo, E, r1[i], r2[i], r3[i], &eb, &ab);
+  else if (kind == 2) v = (double)sample_kn(T.kn, E, r1[i], r2[i], r3[i], &eb, &ab);
+  else if (kind == 3) v = 1.0 - (double)sample_gs_omu(T.gs, E, r1[i], r2[i], &eb, &ab);
+  out[i] = v;
+  ebin[i] = eb;
+  abin[i] = ab;
+}
+
+__global__ void k_eval(Tables<R> T, int which, int n, const double* e, const int* imat, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R E = (R)e[i];
+  const int m = imat ? imat[i] : 0;
+  R v = 0;
+  switch (which) {
+    case 0: v = spline_value(T.itr1, E, m); break;
+    case 1: v = spline_value(T.dedx, E, m); break;
+    case 2: v = spline_value(T.brem, E, m); break;
+    case 3: v = spline_value(T.moller, E, 0); break;
+    case 4: v = spline_value(T.k1, E, 0); break;
+    case 5: v = linear_value(T.gmax, E, 0); break;
+    case 6: v = linear_value(T.ptot, E, m); break;
+    case 7: v = linear_value(T.pcomp, E, m); break;
+    case 8: v = linear_value(T.ppair, E, m); break;
+    default: break;
+  }
+  out[i] = (double)v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host side of this precision)
+// ---------------------------------------------------------------------------
+#define DPM_CAT2(a, b) a##b
+#define DPM_CAT(a, b) DPM_CAT2(a, b)
+#if DPM_PREC == 32
+#define DPM_SFX _f32
+#else
+#define DPM_SFX _f64
+#endif
+
+hipError_t DPM_CAT(launch_source, DPM_SFX)(const KParams<R>& P, const SourceParams& S, uint64_t h0, int n,
+                                           const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_source, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, S, h0, n, qe, qg);
+  return hipGetLastError();
+}
+
+hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in,
+                                            int* head, const QueueView<R>& qe, const QueueView<R>& qg,
+                                            hipStream_t st) {
+  if (geom == GEOM_SLAB)
+    hipLaunchKernelGGL(k_photons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+  else
+    hipLaunchKernelGGL(k_photons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+  return hipGetLastError();
+}
+
+hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in,
+                                              int* head, const QueueView<R>& qe, const QueueView<R>& qg,
+                                              hipStream_t st) {
+  if (geom == GEOM_SLAB)
+    hipLaunchKernelGGL(k_electrons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+  else
+    hipLaunchKernelGGL(k_electrons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+  return hipGetLastError();
+}
+
+hipError_t DPM_CAT(launch_sample, DPM_SFX)(const Tables<R>& T, int kind, int n, const double* e, const int* imat,
+                                           const uint32_t* r1, const uint32_t* r2, const uint32_t* r3, double* out,
+                                           int* ebin, int* abin, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, st, T, kind, n, e, imat, r1, r2, r3, out, ebin,
+                     abin);
+  return hipGetLastError();
+}
+
+hipError_t DPM_CAT(launch_eval, DPM_SFX)(const Tables<R>& T, int which, int n, const double* e, const int* imat,
+                                         double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval, dim3((n + 255) / 256), dim3(256), 0, st, T, which, n, e, imat, out);
+  return hipGetLastError();
+}
+
+}  // namespace dpm

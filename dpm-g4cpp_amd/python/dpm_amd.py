@@ -1,0 +1,214 @@
+"""ctypes binding of the MI355X DPM engine C ABI (include/dpm_amd.h).
+
+Loads ``dpm-g4cpp_amd/lib/libdpm_amd.so``, built in-tree by ``make -C
+dpm-g4cpp_amd``; every call goes through the C ABI and raises
+``DpmError`` carrying ``dpm_last_error()`` on failure. There is no CPU
+fallback: without the library (or without a GPU) the calls fail.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libdpm_amd.so")
+CLI_PATH = os.path.join(PKG, "bin", "dpm_Simulate")
+
+DPM_FP32, DPM_FP64 = 0, 1
+ELECTRON, PHOTON, POSITRON = -1, 0, 1
+SRC_PENCIL, SRC_FIELD = 0, 1
+COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries", "e_steps", "g_steps")
+
+EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
+            "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
+            "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_last_error", "dpm_abi_version")
+
+
+class DpmError(RuntimeError):
+    pass
+
+
+class Source(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("particle", ctypes.c_int), ("energy", ctypes.c_double),
+                ("emin", ctypes.c_double), ("emax", ctypes.c_double), ("x0", ctypes.c_double),
+                ("y0", ctypes.c_double), ("field_x", ctypes.c_double), ("field_y", ctypes.c_double)]
+
+
+class RunOptions(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("hist_begin", ctypes.c_uint64), ("precision", ctypes.c_int),
+                ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
+                ("hist_edep", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("e_scored", ctypes.c_double), ("e_escaped", ctypes.c_double),
+                ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 9),
+                ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
+                ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 9),
+                ("counters_electron", ctypes.c_uint64 * 9)]
+
+    def as_dict(self):
+        return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped, n_pair=self.n_pair,
+                    n_annih=self.n_annih, counters=dict(zip(COUNTER_NAMES, list(self.counters))),
+                    counters_photon=dict(zip(COUNTER_NAMES, list(self.counters_photon))),
+                    counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
+                    rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
+                    kernel_calls=list(self.kernel_calls))
+
+
+class Beamlet(ctypes.Structure):
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("particle", ctypes.c_int),
+                ("energy", ctypes.c_double), ("nhist", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DpmError("libdpm_amd.so not built: run `make -C dpm-g4cpp_amd` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, i32, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_double
+        L.dpm_last_error.restype = ctypes.c_char_p
+        L.dpm_tables_load.argtypes = [ctypes.c_char_p, ip, ctypes.POINTER(vp)]
+        L.dpm_tables_destroy.argtypes = [vp]
+        L.dpm_tables_num_materials.argtypes = [vp]
+        L.dpm_phantom_create_slab.argtypes = [vp, dp, ip, ctypes.POINTER(vp)]
+        L.dpm_phantom_create_voxel.argtypes = [vp, ip, ip, ip, dp, vp, vp, ctypes.POINTER(vp)]
+        L.dpm_phantom_destroy.argtypes = [vp]
+        L.dpm_phantom_num_bins.argtypes = [vp]
+        L.dpm_phantom_num_bins.restype = ctypes.c_int64
+        L.dpm_simulate.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ctypes.POINTER(RunOptions), vp,
+                                   vp, ctypes.POINTER(Stats)]
+        L.dpm_simulate_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.POINTER(RunOptions), dp, vp,
+                                            ctypes.POINTER(Stats)]
+        L.dpm_beamlets_fetch.argtypes = [vp, vp, vp]
+        L.dpm_sample.argtypes = [vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DpmError("dpm error %d: %s" % (rc, lib().dpm_last_error().decode()))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Tables:
+    """Device table pack loaded from a reference-format data directory."""
+
+    def __init__(self, datadir, device=0):
+        self.h = ctypes.c_void_p()
+        _check(lib().dpm_tables_load(datadir.encode(), device, ctypes.byref(self.h)))
+        self.nmat = lib().dpm_tables_num_materials(self.h)
+
+    def close(self):
+        if self.h:
+            lib().dpm_tables_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sample(self, kind, energy, r1, r2, r3=None, imat=None, precision=DPM_FP32):
+        """Sampler with explicit 24-bit integer uniforms; kind in sb|mo|kn|gs."""
+        k = {"sb": 0, "mo": 1, "kn": 2, "gs": 3}[kind]
+        e = np.ascontiguousarray(energy, dtype=np.float64)
+        n = e.size
+        u1 = np.ascontiguousarray(r1, dtype=np.uint32)
+        u2 = np.ascontiguousarray(r2, dtype=np.uint32)
+        u3 = None if r3 is None else np.ascontiguousarray(r3, dtype=np.uint32)
+        im = None if imat is None else np.ascontiguousarray(imat, dtype=np.int32)
+        out = np.zeros(n)
+        eb = np.zeros(n, dtype=np.int32)
+        ab = np.zeros(n, dtype=np.int32)
+        _check(lib().dpm_sample(self.h, k, precision, n, _ptr(e), _ptr(im), _ptr(u1), _ptr(u2), _ptr(u3), _ptr(out),
+                                _ptr(eb), _ptr(ab)))
+        return out, eb, ab
+
+    def eval(self, which, energy, imat=None, precision=DPM_FP32):
+        e = np.ascontiguousarray(energy, dtype=np.float64)
+        im = None if imat is None else np.ascontiguousarray(imat, dtype=np.int32)
+        out = np.zeros(e.size)
+        _check(lib().dpm_eval(self.h, which, precision, e.size, _ptr(e), _ptr(im), _ptr(out)))
+        return out
+
+
+class Phantom:
+    def __init__(self, tables, h, shape):
+        self.tables = tables
+        self.h = h
+        self.shape = shape
+        self.nbins = lib().dpm_phantom_num_bins(h)
+
+    @classmethod
+    def slab(cls, tables, lbox, geom_index):
+        h = ctypes.c_void_p()
+        _check(lib().dpm_phantom_create_slab(tables.h, lbox, geom_index, ctypes.byref(h)))
+        return cls(tables, h, None)
+
+    @classmethod
+    def voxel(cls, tables, mat, rho, lbox):
+        mat = np.ascontiguousarray(mat, dtype=np.uint8)
+        rho = np.ascontiguousarray(rho, dtype=np.float32)
+        nz, ny, nx = mat.shape
+        h = ctypes.c_void_p()
+        _check(lib().dpm_phantom_create_voxel(tables.h, nx, ny, nz, lbox, _ptr(mat), _ptr(rho), ctypes.byref(h)))
+        return cls(tables, h, mat.shape)
+
+    def close(self):
+        if self.h:
+            lib().dpm_phantom_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
+             seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False):
+    """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None)."""
+    src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
+                 x0=x0, y0=y0, field_x=field, field_y=field)
+    he = np.zeros(nhist) if per_history else None
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
+                     waves_per_cu=waves_per_cu, hist_edep=_ptr(he))
+    tally = np.zeros(phantom.nbins)
+    steps = np.zeros(phantom.nbins)
+    st = Stats()
+    _check(lib().dpm_simulate(tables.h, phantom.h, ctypes.byref(src), nhist, ctypes.byref(opt), _ptr(tally),
+                              _ptr(steps), ctypes.byref(st)))
+    if phantom.shape is not None:
+        tally = tally.reshape(phantom.shape)
+        steps = steps.reshape(phantom.shape)
+    return tally, steps, st.as_dict(), he
+
+
+def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
+                      waves_per_cu=0):
+    """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
+    arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
+                                       for b in beamlets])
+    opt = RunOptions(seed=seed, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu)
+    row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
+    st = Stats()
+    _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,
+                                       _ptr(row_ptr), ctypes.byref(st)))
+    nnz = int(row_ptr[-1])
+    col = np.zeros(nnz, dtype=np.int32)
+    val = np.zeros(nnz)
+    _check(lib().dpm_beamlets_fetch(tables.h, _ptr(col), _ptr(val)))
+    return row_ptr, col, val, st.as_dict()

@@ -1,0 +1,227 @@
+/*
+ * dpm_amd.h -- C ABI of the MI355X DPM transport engine (libdpm_amd.so).
+ *
+ * The engine re-implements, as gfx950 HIP kernels, the per-history coupled
+ * electron-photon transport of the reference's dpm_Simulate:
+ *   Simulate(...)                   Simulation/inc/SimDPMLike.hh:46, SimDPMLike.cc:40-251
+ * called from the reference CLI at dpm_Simulate.cc:82 after the three data
+ * objects were loaded (dpm_Simulate.cc:71-78). Everything crossing this ABI is
+ * plain C: pointers, sizes and status codes. No C++ exception crosses it.
+ *
+ * Conventions
+ *  - Every function returns DPM_OK (0) or a negative DPM_E* code; the text of
+ *    the last error of the calling thread is dpm_last_error().
+ *  - Handles are thread-compatible, not thread-safe: one thread at a time per
+ *    handle. Each tables handle owns one HIP stream and the scratch buffers
+ *    of the runs made with it.
+ *  - Lengths are in mm, energies in MeV, densities in g/cm3 (as the reference).
+ */
+#ifndef DPM_AMD_H
+#define DPM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPM_ABI_VERSION 1
+
+enum {
+  DPM_OK = 0,
+  DPM_E_INVALID = -1,   /* bad argument */
+  DPM_E_IO = -2,        /* table file missing or malformed */
+  DPM_E_DEVICE = -3,    /* HIP runtime error / no device */
+  DPM_E_OVERFLOW = -4,  /* a particle queue overflowed (reduce histories per chunk) */
+  DPM_E_NOMEM = -5
+};
+
+enum { DPM_FP32 = 0, DPM_FP64 = 1 };              /* transport arithmetic */
+enum { DPM_ELECTRON = -1, DPM_PHOTON = 0, DPM_POSITRON = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Tables. The descriptor carries the raw node data of the reference's
+ * run-time table classes; the library derives the interpolation coefficients
+ * (as SimDataSpline::SetUp does, SimDataSpline.cc:33-67) and the device packs.
+ * ------------------------------------------------------------------------- */
+typedef struct dpm_spline_desc {   /* one log-grid spline: SimDataSpline (SimDataSpline.hh:38-50) */
+  int n;                           /* nodes */
+  const double* x;                 /* [n] energies, log-spaced */
+  const double* y;                 /* [n] values */
+} dpm_spline_desc;
+
+typedef struct dpm_linear_desc {   /* one linear-grid table: SimDataLinear (SimDataLinear.hh:38-45) */
+  int n;
+  const double* x;
+  const double* y;
+} dpm_linear_desc;
+
+typedef struct dpm_alias_desc {    /* a family of linear-pdf alias tables: SimLinAliasData */
+  int nmat;                        /* materials (1 for Moller/KN) */
+  int ne;                          /* primary energies (log grid) */
+  int ns;                          /* points per table */
+  const double* energy;            /* [ne] */
+  const double* x;                 /* [nmat*ne*ns] */
+  const double* y;                 /* [nmat*ne*ns] */
+  const double* w;                 /* [nmat*ne*ns] alias keep-probability */
+  const int*    alias;             /* [nmat*ne*ns] alias index */
+} dpm_alias_desc;
+
+typedef struct dpm_gs_desc {       /* SimGSTables (SimGSTables.cc:19-80), reference material */
+  int ne, ns;
+  const double* energy;            /* [ne] */
+  const double* transf;            /* [ne] transformation parameter a(E_i) */
+  const double* u;                 /* [ne*ns] */
+  const double* a;                 /* [ne*ns] */
+  const double* b;                 /* [ne*ns] */
+} dpm_gs_desc;
+
+typedef struct dpm_table_desc {
+  int nmat;
+  double electron_cut, gamma_cut;          /* SimMaterialData::fElectronCut/fGammaCut */
+  const double* density;                   /* [nmat] SimMaterialData::fMaterialDensity */
+  const double* moller_scale;              /* [nmat] SimMaterialData::fMollerIMFPScaling */
+  const dpm_spline_desc* itr1;             /* [nmat] SimITr1MFPElastic (scattering-power corrected) */
+  const dpm_spline_desc* dedx;             /* [nmat] SimStoppingPower */
+  const dpm_spline_desc* brem;             /* [nmat] SimIMFPBrem */
+  dpm_spline_desc moller;                  /* SimIMFPMoller (reference material) */
+  dpm_spline_desc k1;                      /* SimMaxScatStrength (reference material) */
+  dpm_linear_desc gmax;                    /* SimIMFPMaxPhoton */
+  const dpm_linear_desc* ptotal;           /* [nmat] SimIMFPPhoton(0) */
+  const dpm_linear_desc* pcompton;         /* [nmat] SimIMFPPhoton(1) */
+  const dpm_linear_desc* ppair;            /* [nmat] SimIMFPPhoton(2) */
+  dpm_alias_desc sb;                       /* SimSBTables */
+  dpm_alias_desc moller_tab;               /* SimMollerTables */
+  dpm_alias_desc kn;                       /* SimKNTables */
+  dpm_gs_desc gs;                          /* SimGSTables */
+} dpm_table_desc;
+
+typedef struct dpm_tables dpm_tables;
+
+/* Replaces SimElectronData::Load + SimPhotonData::Load + SimMaterialData::Load
+ * (SimElectronData.cc:44-115, SimPhotonData.cc:28-78, SimMaterialData.cc:6-53)
+ * followed by the upload of their content to the device. */
+int  dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out);
+/* Same, reading the reference's 14 table files in `dir` with the host readers
+ * (identical file names, comment-line counts and fields). */
+int  dpm_tables_load(const char* dir, int device, dpm_tables** out);
+void dpm_tables_destroy(dpm_tables* t);
+int  dpm_tables_num_materials(const dpm_tables* t);
+
+/* ---------------------------------------------------------------------------
+ * Geometry. Replaces Geom (Simulation/src/Geom.cc, Geom.hh:52-101).
+ * ------------------------------------------------------------------------- */
+typedef struct dpm_phantom dpm_phantom;
+
+/* The reference's pre-defined slab geometry: cubic boxes of size `lbox`,
+ * material by depth index per `geom_index` 0..3 (Geom.cc:133-164), lateral
+ * extent +-100 mm, depth extent [-lbox, 100] mm (Geom.cc:25), depth-dose
+ * scoring (Geom.cc:120-130). */
+int  dpm_phantom_create_slab(const dpm_tables* t, double lbox, int geom_index, dpm_phantom** out);
+/* A voxel phantom of nx*ny*nz cubic voxels of size `lbox`, centred on the
+ * origin; the source plane is its -z face. `mat` (material indices) and `rho`
+ * (g/cm3) are [nz][ny][nx] (x fastest). Leaving the box ends a track. */
+int  dpm_phantom_create_voxel(const dpm_tables* t, int nx, int ny, int nz, double lbox,
+                              const uint8_t* mat, const float* rho, dpm_phantom** out);
+void dpm_phantom_destroy(dpm_phantom* p);
+/* number of tally bins: depth bins (slab) or voxels */
+int64_t dpm_phantom_num_bins(const dpm_phantom* p);
+
+/* ---------------------------------------------------------------------------
+ * Sources and runs
+ * ------------------------------------------------------------------------- */
+enum { DPM_SRC_PENCIL = 0, DPM_SRC_FIELD = 1 };
+
+typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
+  int kind;                        /* DPM_SRC_* */
+  int particle;                    /* DPM_ELECTRON or DPM_PHOTON */
+  double energy;                   /* MeV (mono-energetic when emax <= emin) */
+  double emin, emax;               /* uniform spectrum [emin, emax] when emax > emin */
+  double x0, y0;                   /* entry point / field centre [mm] */
+  double field_x, field_y;         /* field size [mm] (DPM_SRC_FIELD) */
+} dpm_source;
+
+typedef struct dpm_run_options {
+  uint64_t seed;                   /* Philox key */
+  uint64_t hist_begin;             /* id of the first history (stream selection) */
+  int precision;                   /* DPM_FP32 (default) or DPM_FP64 */
+  int nbatch;                      /* >1: per-batch sum and sum of squares of the voxel tally */
+  int64_t chunk;                   /* histories per queue fill; 0 = automatic */
+  int waves_per_cu;                /* persistent waves per CU; 0 = automatic */
+  double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
+} dpm_run_options;
+
+enum {
+  DPM_CNT_VOXEL_VISITS = 0, DPM_CNT_DEPOSITS, DPM_CNT_SB, DPM_CNT_MOLLER, DPM_CNT_KN, DPM_CNT_GS,
+  DPM_CNT_SECONDARIES, DPM_CNT_E_STEPS, DPM_CNT_G_STEPS, DPM_NCNT
+};
+
+typedef struct dpm_stats {
+  double seconds;                  /* device time of the run (events on the run stream) */
+  double e_scored, e_escaped;      /* MeV */
+  uint64_t n_pair, n_annih;
+  uint64_t counters[DPM_NCNT];     /* byte-model counters */
+  int rounds;                      /* queue rounds executed */
+  int hist_size;                   /* slab: rows written by the reference rule */
+  double kernel_ms[4];             /* device time in source, photon, electron, tally-fold kernels */
+  int kernel_calls[4];             /* launches of each of those kernels */
+  uint64_t counters_photon[DPM_NCNT];    /* counters of the photon kernel */
+  uint64_t counters_electron[DPM_NCNT];  /* counters of the electron kernel */
+} dpm_stats;
+
+/* Replaces Simulate(nprimary, eprimary, iselectron, lbox, mat, el, ph,
+ * geomIndex, filename) (SimDPMLike.hh:46). Runs `nhist` histories of the
+ * source in the phantom and returns the tally in caller-owned host arrays:
+ *   slab:  tally[nbins] = energy per depth bin (MeV, summed over histories),
+ *          steps[nbins] = number of scoring calls per bin (may be NULL);
+ *   voxel: tally[nbins] = energy per voxel (MeV); with nbatch>1, steps (may be
+ *          NULL) receives the per-voxel sum over batches of the squared batch
+ *          tally. */
+int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
+                 const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
+
+/* Batched beamlets: `nb` pencil beamlets (entry point, particle, energy,
+ * history count) in one run, scored into a sparse dose-influence matrix:
+ * for each beamlet the voxels above `rel_threshold` of its maximum, as CSR
+ * (row_ptr[nb+1]; nnz = row_ptr[nb]). The column indices and values stay on
+ * the device in the tables handle until dpm_beamlets_fetch copies them into
+ * caller arrays of nnz entries (col: voxel index, val: MeV). */
+typedef struct dpm_beamlet {
+  double x0, y0;
+  int particle;
+  double energy;
+  uint64_t nhist;
+} dpm_beamlet;
+
+int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb,
+                          const dpm_run_options* opt, double rel_threshold,
+                          int64_t* row_ptr, dpm_stats* stats);
+int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val);
+
+/* ---------------------------------------------------------------------------
+ * Interaction samplers with explicit uniforms (the reference's own model
+ * tests drive PerformBrem/PerformMoller/PerformMSCAngularDeflection this way,
+ * testAPI.cc:92-199). Uniforms are 24-bit integers k (uniform = k * 2^-24).
+ *   kind 0 SB    : SimSBTables::SampleEnergyTransfer(E, imat, r1, r2, r3)   SimSBTables.cc:93
+ *   kind 1 Moller: SimMollerTables::SampleEnergyTransfer(E, r1, r2, r3)    SimMollerTables.cc:69
+ *   kind 2 KN    : SimKNTables::SampleEnergyTransfer(E, r1, r2, r3)        SimKNTables.cc:69
+ *   kind 3 GS    : SimGSTables::SampleAngularDeflection(E, r1, r2)         SimGSTables.cc:84
+ * out[i] is the sampled value; ebin[i] the chosen energy-grid table and
+ * abin[i] the alias (or cumulative) bin, for exactness checks. */
+int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* energy, const int* imat,
+               const uint32_t* r1, const uint32_t* r2, const uint32_t* r3,
+               double* out, int32_t* ebin, int32_t* abin);
+
+/* The interpolations SimDPMLike.cc uses, on the device:
+ *   which 0 ITr1(E,m) 1 dE/dx(E,m) 2 brem IMFP(E,m) 3 Moller IMFP(E) 4 K1(E)
+ *         5 max total IMFP(E) 6 total(E,m) 7 Compton(E,m) 8 pair(E,m) */
+int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energy, const int* imat, double* out);
+
+const char* dpm_last_error(void);
+int dpm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPM_AMD_H */

@@ -27,6 +27,8 @@
 namespace {
 
 std::string g_err;
+// test hook: 1 = drop the Compton electron (planted mistake for the tests)
+int g_fault = 0;
 
 // ---------------------------------------------------------------------------
 // Spline with log-spaced nodes: Simulation/inc/SimDataSpline.hh:38-50,
@@ -890,6 +892,8 @@ struct Sim {
         const double elEner = tr.ekin - phEner;
         if (elEner < ecut) {
           geo.score(elEner, tr.box);
+        } else if (g_fault == 1) {
+          // planted mistake: the Compton electron is lost
         } else {
           const double e0 = tr.ekin * kInvEMC2;
           const double elCost = (1.0 + e0) * std::sqrt((1.0 - theEps) / (e0 * (2.0 + e0 * (1.0 - theEps))));
@@ -1150,5 +1154,7 @@ double oracle_eval(int which, double e, int imat) {
 }
 
 void oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox10(ctr, key, out); }
+
+void oracle_set_fault(int f) { g_fault = f; }
 
 }  // extern "C"

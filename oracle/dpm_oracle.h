@@ -78,6 +78,8 @@ double oracle_sample_kn(double e, double r1, double r2, double r3, int* ebin, in
 double oracle_sample_gs(double e, double r1, double r2, int* ebin, int* cbin);
 /* table interpolations as used by the transport (with the reference clamps) */
 double oracle_eval(int which, double e, int imat);
+/* test hook: plant a known mistake (1 = drop the Compton electron), 0 = off */
+void   oracle_set_fault(int f);
 /* Philox4x32-10 block for tests */
 void   oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -146,6 +146,11 @@ def evaluate(which, e, imat=0):
     return lib().oracle_eval(which, e, imat)
 
 
+def set_fault(f):
+    """Plant a known mistake in the oracle (1 = drop the Compton electron)."""
+    lib().oracle_set_fault(int(f))
+
+
 def philox(ctr, key):
     c = (ctypes.c_uint32 * 4)(*ctr)
     k = (ctypes.c_uint32 * 2)(*key)

@@ -1,0 +1,62 @@
+"""Invariants of the oracle that the mathematics fixes (no reference needed).
+
+* energy conservation per run: scored + escaped = sum of primary energies
+  + 2 mc^2 (annihilations - pairs), for electrons and photons, slab and voxel;
+* Philox-mode results do not depend on history order or chunking: a run of
+  histories [0, N) equals the concatenation of runs [0, k) and [k, N);
+* a planted mistake in the oracle (dropping the Compton electron) must break
+  energy conservation -- the balance check is sensitive.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+
+EMC2 = 0.510991
+
+
+def balance(r, total_in):
+    return (r["e_scored"] + r["e_escaped"]) - (total_in + 2 * EMC2 * (r["n_annih"] - r["n_pair"]))
+
+
+@pytest.mark.parametrize("ptype,e0", [(-1, 15.0), (0, 12.0), (-1, 3.0), (0, 0.8)])
+def test_slab_energy_conservation(tables3, ptype, e0):
+    n = 3000
+    r = oracle_lib.run_slab(tables3, n, e0=e0, ptype=ptype, rng=oracle_lib.RNG_PHILOX, seed=17)
+    assert abs(balance(r, n * e0)) < 1e-9 * n * e0
+
+
+def test_voxel_energy_conservation_and_tally(tables3):
+    rng = np.random.default_rng(0)
+    mat = rng.integers(0, 3, size=(12, 10, 8)).astype(np.uint8)
+    rho = np.array([1.0, 4.54, 1.85], dtype=np.float32)[mat] * rng.uniform(0.9, 1.1, mat.shape).astype(np.float32)
+    scale = float(np.max(rho / np.array([1.0, 4.54, 1.85])[mat]))
+    n = 1500
+    r = oracle_lib.run_voxel(tables3, mat, rho, 2.0, n, ptype=-1, e0=6.0, field=10.0, seed=5, majorant_scale=scale)
+    assert abs(balance(r, n * 6.0)) < 1e-9 * n * 6.0
+    assert abs(r["edep"].sum() - r["e_scored"]) < 1e-9 * r["e_scored"]
+
+
+def test_history_order_independence(tables3):
+    a = oracle_lib.run_slab(tables3, 400, rng=oracle_lib.RNG_PHILOX, seed=9, per_history=True)
+    b1 = oracle_lib.run_slab(tables3, 150, rng=oracle_lib.RNG_PHILOX, seed=9, per_history=True)
+    b2 = oracle_lib.run_slab(tables3, 250, rng=oracle_lib.RNG_PHILOX, seed=9, hist_begin=150, per_history=True)
+    assert np.array_equal(a["hist_edep"], np.concatenate([b1["hist_edep"], b2["hist_edep"]]))
+
+
+def test_batch_statistics(tables0):
+    mat = np.zeros((20, 20, 20), np.uint8)
+    rho = np.ones((20, 20, 20), np.float32)
+    r = oracle_lib.run_voxel(tables0, mat, rho, 2.0, 4000, ptype=0, emin=0.5, emax=20.0, field=30.0, seed=1, nbatch=8)
+    np.testing.assert_allclose(r["batch_sum"], r["edep"].ravel(), rtol=1e-12, atol=1e-12)
+    assert np.all(r["batch_sum2"] * 8 + 1e-12 >= r["batch_sum"] ** 2 * (1 - 1e-12))
+
+
+def test_planted_mistake_breaks_conservation(tables3):
+    n = 2000
+    oracle_lib.set_fault(1)
+    try:
+        r = oracle_lib.run_slab(tables3, n, e0=5.0, ptype=0, rng=oracle_lib.RNG_PHILOX, seed=17)
+    finally:
+        oracle_lib.set_fault(0)
+    assert abs(balance(r, n * 5.0)) > 1e-3 * n * 5.0
