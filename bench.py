@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-gamma", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="first run a plain device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU split (history ranges per GPU, one FP64 "
@@ -85,6 +87,15 @@ def main():
     import dpm_amd
     import gamma
 
+    if args.copy_probe > 0:
+        import torch
+        n = int(args.copy_probe * (1 << 30)) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}), flush=True)
+        del src, dst
     tables0 = __graft_entry__.tables_dir(0)
     nx, lbox, field, emin, emax = 150, 2.0, 100.0, 0.5, 20.0
     nhist = int(args.histories)

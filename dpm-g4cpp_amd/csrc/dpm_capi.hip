@@ -605,6 +605,7 @@ int finish_stats(dpm_tables* t, dpm_stats* st) {
     st->n_pair = h.cnt[0][kCntPair] + h.cnt[1][kCntPair];
     st->n_annih = h.cnt[0][kCntAnnih] + h.cnt[1][kCntAnnih];
     st->e_escaped = h.escaped;
+    st->e_primary = h.e_primary;
     st->hist_size = std::max(20, h.maxiz + 10);
   }
   return DPM_OK;
@@ -925,7 +926,8 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
     hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr, d_cnt,
                        (int32_t*)nullptr, (double*)nullptr, 0);
     std::vector<int64_t> cnt(nb);
-    if (hipMemcpy(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+    // the run stream is non-blocking: wait for the count pass before the copy
+    if (hipStreamSynchronize(s) != hipSuccess || hipMemcpy(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost) != hipSuccess)
       rc = set_error(DPM_E_DEVICE, "CSR count failed");
     if (rc == DPM_OK) {
       row_ptr[0] = 0;

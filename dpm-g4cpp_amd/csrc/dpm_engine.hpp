@@ -19,6 +19,7 @@ enum {
 struct StatsDev {
   unsigned long long cnt[2][kNumCounters];
   double escaped;
+  double e_primary;
   int maxiz;
   int pad;
 };

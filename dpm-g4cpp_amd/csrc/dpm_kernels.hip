@@ -169,6 +169,12 @@ __device__ __forceinline__ void load_track(const QueueView<R>& q, int i, const K
   t.rng.shi = q.shi[i];
   t.rng.ndraw = (tt >> 2) & 63u;
   t.rng.nchild = 0;
+  // a stream resumed mid-block (draws taken by the source) needs its block
+  if (t.rng.ndraw & 3u) {
+    uint32_t o[4];
+    Philox::block(t.rng.ndraw >> 2, t.rng.slo, t.rng.shi, t.rng.hist, t.rng.k0, t.rng.k1, o);
+    t.rng.b0 = o[0]; t.rng.b1 = o[1]; t.rng.b2 = o[2]; t.rng.b3 = o[3];
+  }
 }
 
 // PerformAnnihilation (SimDPMLike.cc:753-791): two back-to-back 0.511 photons
@@ -193,8 +199,8 @@ __device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Tra
 __global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S, uint64_t h0, int n,
                                                    QueueView<R> qe, QueueView<R> qg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t hid = h0 + (uint64_t)i;
+  const bool valid = i < n;
+  const uint64_t hid = h0 + (uint64_t)(valid ? i : 0);
   Rng rng;
   rng.k0 = P.k0; rng.k1 = P.k1;
   rng.hist = (uint32_t)hid;
@@ -220,15 +226,22 @@ __global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S,
   }
   R E = (R)e;
   if (S.emax > S.emin) E = (R)S.emin + rng.uni<R>() * (R)(S.emax - S.emin);
+  // sum of primary energies: one atomic per wave (all lanes take part)
+  double ew = valid ? (double)E : 0.0;
+  for (int o = 32; o > 0; o >>= 1) ew += __shfl_xor(ew, o);
+  if (lane_id() == 0 && ew != 0.0) unsafeAtomicAdd(&P.stats->e_primary, ew);
+  if (!valid) return;
   uint32_t tag = 0;
   if (S.nbeamlet > 0) tag = (uint32_t)bl;
   else if (S.nbatch > 1) {
     const uint64_t rel = hid - S.hist_begin;
     tag = (uint32_t)((rel * (uint64_t)S.nbatch) / S.nhist_total) - S.batch_first;
   }
-  const QueueView<R>& q = particle == 0 ? qg : qe;
   const int type = particle == 0 ? 0 : (particle < 0 ? 1 : 2);
-  const int slot = atomicAdd(q.count, 1);
+  const QueueView<R>& q = particle == 0 ? qg : qe;
+  int slot;
+  if (particle == 0) slot = wave_reserve(qg.count);
+  else slot = wave_reserve(qe.count);
   if (slot >= q.cap) { atomicAdd(P.overflow, 1); return; }
   q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
   q.u[slot] = (R)0.0; q.v[slot] = (R)0.0; q.w[slot] = (R)1.0;

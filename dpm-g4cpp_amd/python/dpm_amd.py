@@ -42,13 +42,14 @@ class RunOptions(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("e_scored", ctypes.c_double), ("e_escaped", ctypes.c_double),
-                ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 9),
+                ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 9),
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 9),
                 ("counters_electron", ctypes.c_uint64 * 9)]
 
     def as_dict(self):
-        return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped, n_pair=self.n_pair,
+        return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
+                    e_primary=self.e_primary, n_pair=self.n_pair,
                     n_annih=self.n_annih, counters=dict(zip(COUNTER_NAMES, list(self.counters))),
                     counters_photon=dict(zip(COUNTER_NAMES, list(self.counters_photon))),
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),

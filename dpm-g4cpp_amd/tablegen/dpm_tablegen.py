@@ -789,6 +789,23 @@ def generate(outdir, index, verbose=True):
     return outdir
 
 
+TABLE_FILES = ("mat.dat", "el_itr1mfp.dat", "el_scatStrength.dat", "el_GSDtrData.dat", "eloss_rdedx.dat",
+               "imfp_brem.dat", "imfp_moller.dat", "ioni_MollerDtrData.dat", "brem_SBDtrData.dat",
+               "imfp_globalMax.dat", "imfp_total.dat", "imfp_compton.dat", "imfp_pairp.dat",
+               "compton_KNDtrData.dat")
+
+
+def tables_digest(d):
+    """sha256 over the 14 run-time table files (names and bytes)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in TABLE_FILES:
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(f.encode())
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def ensure_tables(outdir, index):
     """Generate the tables into ``outdir`` unless a complete set is there."""
     stamp = os.path.join(outdir, ".complete")

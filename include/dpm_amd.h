@@ -160,6 +160,7 @@ enum {
 typedef struct dpm_stats {
   double seconds;                  /* device time of the run (events on the run stream) */
   double e_scored, e_escaped;      /* MeV */
+  double e_primary;                /* MeV, sum of the primaries' kinetic energies */
   uint64_t n_pair, n_annih;
   uint64_t counters[DPM_NCNT];     /* byte-model counters */
   int rounds;                      /* queue rounds executed */

@@ -12,7 +12,6 @@ sources with `make -C oracle ref` into oracle/_ref/, never copying them):
   uniforms lie on the 2^-24 lattice, as exact float64 values;
 * tables_cfg3.sha256 -- digest of the table files these were computed on.
 """
-import hashlib
 import os
 import subprocess
 import sys
@@ -24,19 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "dpm-g4cpp_amd", "tablegen"))
 import dpm_tablegen  # noqa: E402
 
-TABLE_FILES = ("mat.dat", "el_itr1mfp.dat", "el_scatStrength.dat", "el_GSDtrData.dat", "eloss_rdedx.dat",
-               "imfp_brem.dat", "imfp_moller.dat", "ioni_MollerDtrData.dat", "brem_SBDtrData.dat",
-               "imfp_globalMax.dat", "imfp_total.dat", "imfp_compton.dat", "imfp_pairp.dat",
-               "compton_KNDtrData.dat")
-
-
-def tables_digest(d):
-    h = hashlib.sha256()
-    for f in TABLE_FILES:
-        with open(os.path.join(d, f), "rb") as fh:
-            h.update(f.encode())
-            h.update(fh.read())
-    return h.hexdigest()
+from dpm_tablegen import tables_digest  # noqa: E402
 
 
 def lattice(rng, n):

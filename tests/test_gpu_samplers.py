@@ -22,7 +22,7 @@ KINDS = ("sb", "mo", "kn", "gs")
 
 @pytest.fixture(scope="module")
 def gold(tables3):
-    from golden.make_golden import tables_digest
+    from dpm_tablegen import tables_digest
     with open(os.path.join(HERE, "golden", "tables_cfg3.sha256")) as f:
         assert tables_digest(tables3) == f.read().strip(), "generated tables differ from the fixtures' tables"
     return np.load(GOLD)

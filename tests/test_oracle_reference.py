@@ -19,7 +19,7 @@ DENS_CFG3 = [1.0, 4.54, 1.85]
 
 @pytest.fixture(scope="module")
 def checked_tables(tables3):
-    from golden.make_golden import tables_digest
+    from dpm_tablegen import tables_digest
     with open(os.path.join(GOLD, "tables_cfg3.sha256")) as f:
         assert tables_digest(tables3) == f.read().strip(), "generated tables differ from the fixtures' tables"
     return tables3
