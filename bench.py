@@ -92,7 +92,9 @@ def main():
         n = int(args.copy_probe * (1 << 30)) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.cuda.synchronize()
+        # an elementwise kernel that reads n floats and writes n floats
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}), flush=True)
         del src, dst

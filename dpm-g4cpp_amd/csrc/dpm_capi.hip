@@ -305,6 +305,30 @@ struct Workspace {
   }
 };
 
+// histories per queue fill when the caller does not choose
+constexpr int64_t kDefaultChunk = (int64_t)1 << 22;
+
+// grow-only device buffer kept by a tables handle across runs
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t want) {
+    if (want <= bytes && p) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, want) != hipSuccess) { p = nullptr; return false; }
+    bytes = want;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  double* d() const { return (double*)p; }
+};
+
 // tally fold: per-batch rows -> sum, sum of squares; rows cleared
 __global__ void k_fold(double* rows, int nrows, int64_t nvox, double* sum, double* sum2) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nvox; v += (int64_t)gridDim.x * blockDim.x) {
@@ -391,6 +415,8 @@ struct dpm_tables {
   int* ctl = nullptr;        // [0..3] queue counts, [4] head, [5] overflow
   StatsDev* stats = nullptr;
   hipEvent_t ev[10] = {};
+  // tally buffers reused across runs
+  DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
   // CSR of the last beamlet run
   int32_t* csr_col = nullptr;
   double* csr_val = nullptr;
@@ -498,13 +524,17 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
   P.overflow = t->ctl + 5;
 
   const int nbatch = (S.nbeamlet == 0 && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
-  int64_t chunk = opt->chunk > 0 ? opt->chunk : (int64_t)1 << 20;
+  int64_t chunk = opt->chunk > 0 ? opt->chunk : kDefaultChunk;
   if (chunk > ((int64_t)1 << 24)) chunk = (int64_t)1 << 24;
-  const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + 65536, (int64_t)1 << 30);
-  int rc = ws.ensure(cap, t->ctl);
-  if (rc != DPM_OK) return rc;
+  chunk = std::min<int64_t>(chunk, (int64_t)nhist);
   const int wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
   const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
+  // secondaries of a chunk, plus the unfilled tails of the per-wave append
+  // blocks (at most one block per wave, output queue and kernel launch)
+  const int64_t holes = (int64_t)2 * blocks * (256 / 64) * 64;
+  const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + holes + 65536, (int64_t)1 << 30);
+  int rc = ws.ensure(cap, t->ctl);
+  if (rc != DPM_OK) return rc;
   S.hist_begin = opt->hist_begin;
   S.nhist_total = nhist;
   S.nbatch = nbatch;
@@ -697,6 +727,7 @@ void dpm_tables_destroy(dpm_tables* t) {
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   t->wsf.release();
   t->wsd.release();
+  for (DevBuf* b : {&t->buf_rows, &t->buf_steps, &t->buf_sum, &t->buf_sum2, &t->buf_he}) b->release();
   if (t->csr_col) (void)hipFree(t->csr_col);
   if (t->csr_val) (void)hipFree(t->csr_val);
   if (t->dev_f) (void)hipFree(t->dev_f);
@@ -792,28 +823,21 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   const int nbatch = (!slab && opt.nbatch > 1) ? opt.nbatch : 0;
   const int64_t nb = slab ? p->nbins : p->nvox;
   // tally rows: one, or the batches of one chunk
-  int64_t chunk = opt.chunk > 0 ? opt.chunk : (int64_t)1 << 20;
+  int64_t chunk = opt.chunk > 0 ? opt.chunk : kDefaultChunk;
   int rows = 1;
   if (nbatch > 0) {
     const uint64_t per = nhist / nbatch + 1;
     rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)chunk / per));
   }
-  double *d_rows = nullptr, *d_steps = nullptr, *d_sum = nullptr, *d_sum2 = nullptr, *d_he = nullptr;
-  auto cleanup = [&]() {
-    if (d_rows) (void)hipFree(d_rows);
-    if (d_steps) (void)hipFree(d_steps);
-    if (d_sum) (void)hipFree(d_sum);
-    if (d_sum2) (void)hipFree(d_sum2);
-    if (d_he) (void)hipFree(d_he);
-  };
-  if (hipMalloc(&d_rows, sizeof(double) * nb * rows) != hipSuccess ||
-      (slab && hipMalloc(&d_steps, sizeof(double) * nb) != hipSuccess) ||
-      (nbatch > 0 && (hipMalloc(&d_sum, sizeof(double) * nb) != hipSuccess ||
-                      hipMalloc(&d_sum2, sizeof(double) * nb) != hipSuccess)) ||
-      (opt.hist_edep && hipMalloc(&d_he, sizeof(double) * nhist) != hipSuccess)) {
-    cleanup();
+  if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (slab && !t->buf_steps.ensure(sizeof(double) * nb)) ||
+      (nbatch > 0 && (!t->buf_sum.ensure(sizeof(double) * nb) || !t->buf_sum2.ensure(sizeof(double) * nb))) ||
+      (opt.hist_edep && !t->buf_he.ensure(sizeof(double) * nhist)))
     return set_error(DPM_E_NOMEM, "cannot allocate the tally");
-  }
+  double* d_rows = t->buf_rows.d();
+  double* d_steps = slab ? t->buf_steps.d() : nullptr;
+  double* d_sum = nbatch > 0 ? t->buf_sum.d() : nullptr;
+  double* d_sum2 = nbatch > 0 ? t->buf_sum2.d() : nullptr;
+  double* d_he = opt.hist_edep ? t->buf_he.d() : nullptr;
   (void)hipMemsetAsync(d_rows, 0, sizeof(double) * nb * rows, s);
   if (d_steps) (void)hipMemsetAsync(d_steps, 0, sizeof(double) * nb, s);
   if (d_sum) (void)hipMemsetAsync(d_sum, 0, sizeof(double) * nb, s);
@@ -840,13 +864,10 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
       rc = set_error(DPM_E_DEVICE, "per-history copy failed");
     if (rc == DPM_OK) {
       double sc = 0;
-      std::vector<double> h(nb);
-      (void)hipMemcpy(h.data(), res, sizeof(double) * nb, hipMemcpyDeviceToHost);
-      for (double v : h) sc += v;
+      for (int64_t k = 0; k < nb; ++k) sc += tally[k];
       local.e_scored = sc;
     }
   }
-  cleanup();
   if (st) *st = local;
   return rc;
 }

@@ -156,6 +156,11 @@ __host__ __device__ __forceinline__ uint64_t child_serial(uint64_t parent, uint3
   return mix64(parent ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)));
 }
 
+// one Philox block for (block, serial, history): out of line so that the ~20
+// draw sites of the transport share one copy of the 10 rounds (code size)
+__device__ __noinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
+                                          uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3);
+
 struct Rng {
   uint32_t k0, k1;        // key (seed)
   uint32_t hist;
@@ -165,11 +170,7 @@ struct Rng {
   uint32_t b0, b1, b2, b3;
   __device__ __forceinline__ uint32_t draw24() {
     const uint32_t j = ndraw & 3u;
-    if (j == 0u) {
-      uint32_t o[4];
-      Philox::block(ndraw >> 2, slo, shi, hist, k0, k1, o);
-      b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
-    }
+    if (j == 0u) philox_block(ndraw >> 2, slo, shi, hist, k0, k1, &b0, &b1, &b2, &b3);
     ++ndraw;
     const uint32_t w = j == 0u ? b0 : (j == 1u ? b1 : (j == 2u ? b2 : b3));
     return w >> 8;
@@ -191,6 +192,10 @@ struct Rng {
 // arithmetic on the 24-bit uniform against an integer form of the keep
 // probability (exact for uniforms on the 2^-24 lattice).
 // ---------------------------------------------------------------------------
+// FP64 re-evaluation of an energy-bin pick that the FP32 value cannot decide;
+// returns the chosen bin and the lower grid index (out of line: rare path)
+__device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, double invLogDelta_d, uint32_t k1);
+
 template <typename R>
 __device__ __forceinline__ int pick_energy_bin(R lpe, R logMinE, R invLogDelta, double logMinE_d, double invLogDelta_d,
                                                R e, uint32_t k1, int* ielow) {
@@ -201,11 +206,9 @@ __device__ __forceinline__ int pick_energy_bin(R lpe, R logMinE, R invLogDelta, 
   if (sizeof(R) == 4) {
     const R d = frac - r1;
     if (rabs(d) < (R)1e-4 || frac < (R)1e-4 || frac > (R)(1.0 - 1e-4)) {
-      const double phd = (log((double)e) - logMinE_d) * invLogDelta_d;
-      ie = (int)phd;
-      if (ielow) *ielow = ie;
-      const double fr = phd - (double)ie;
-      return ((double)k1 * (1.0 / 16777216.0) < fr) ? ie + 1 : ie;
+      const int2 r = pick_energy_bin_fp64((double)e, logMinE_d, invLogDelta_d, k1);
+      if (ielow) *ielow = r.y;
+      return r.x;
     }
   }
   if (ielow) *ielow = ie;

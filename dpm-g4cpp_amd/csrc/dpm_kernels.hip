@@ -27,6 +27,20 @@ namespace dpm {
 using R = DPM_REAL;
 
 constexpr int kBlock = 256;
+
+__device__ __noinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
+                                          uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3) {
+  uint32_t o[4];
+  Philox::block(blk, slo, shi, hist, k0, k1, o);
+  *o0 = o[0]; *o1 = o[1]; *o2 = o[2]; *o3 = o[3];
+}
+
+__device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, double invLogDelta_d, uint32_t k1) {
+  const double phd = (log(e) - logMinE_d) * invLogDelta_d;
+  const int ie = (int)phd;
+  const double fr = phd - (double)ie;
+  return make_int2(((double)k1 * (1.0 / 16777216.0) < fr) ? ie + 1 : ie, ie);
+}
 constexpr int kSlabLds = 1024;         // depth bins privatised in LDS
 constexpr uint32_t kMaxIter = 200000;  // per-particle iteration cap (termination guard)
 
@@ -45,6 +59,44 @@ __device__ __forceinline__ int wave_reserve(int* counter) {
   if (lane_id() == leader) base = atomicAdd(counter, __popcll(act));
   base = __shfl(base, leader);
   return base + (int)__popcll(act & lanes_below());
+}
+
+// Output queues are appended in per-wave blocks of kBlk consecutive slots: a
+// wave takes a new block from the queue counter only when its current block
+// is full, so the shared counter sees one atomic per kBlk secondaries instead
+// of one per wave and loop trip. The block state (base, used) of each wave
+// lives in LDS because appends happen inside divergent branches. Slots a wave
+// reserved but did not fill are marked empty at the end of the kernel and
+// skipped by the consumer.
+constexpr int kBlk = 64;
+constexpr uint32_t kEmptySlot = 3u;   // type code of an unfilled queue slot
+
+template <typename Q>
+struct OutQ {
+  Q q;
+  volatile int* st;   // this wave's [base, used] in LDS
+};
+
+__device__ __forceinline__ int wave_slot(int* counter, volatile int* st) {
+  const uint64_t act = __ballot(1);
+  const int k = __popcll(act);
+  const int rank = (int)__popcll(act & lanes_below());
+  const int leader = __ffsll((long long)act) - 1;
+  const int base = st[0];
+  const int used = st[1];
+  const int room = kBlk - used;
+  int slot;
+  if (k <= room) {
+    slot = base + used + rank;
+    if (lane_id() == leader) st[1] = used + k;
+  } else {
+    int nb = 0;
+    if (lane_id() == leader) nb = atomicAdd(counter, kBlk);
+    nb = __shfl(nb, leader);
+    slot = rank < room ? base + used + rank : nb + (rank - room);
+    if (lane_id() == leader) { st[0] = nb; st[1] = k - room; }
+  }
+  return slot;
 }
 
 // ---------------------------------------------------------------------------
@@ -93,8 +145,12 @@ __device__ __forceinline__ void acc_flush(BlockAcc& a, const KParams<R>& P, bool
   }
 }
 
+// one LDS atomic per wave (active lanes counted with a ballot)
 template <class BlockAcc>
-__device__ __forceinline__ void count(BlockAcc& a, int which) { atomicAdd(&a.cnt[which], 1u); }
+__device__ __forceinline__ void count(BlockAcc& a, int which) {
+  const uint64_t act = __ballot(1);
+  if (lane_id() == __ffsll((long long)act) - 1) atomicAdd(&a.cnt[which], (unsigned)__popcll(act));
+}
 
 // Geom::Score (Geom.cc:120-130) for the slab; voxel tally row `tag`
 template <int GEOM, class BlockAcc>
@@ -132,9 +188,10 @@ __device__ __forceinline__ uint32_t make_tt(int type, uint32_t ndraw, uint32_t t
   return (uint32_t)type | (ndraw << 2) | (tag << 8);
 }
 
-__device__ __forceinline__ void push(const QueueView<R>& q, int* overflow, const R pos[3], R u, R v, R w, R e,
+__device__ __forceinline__ void push(const OutQ<QueueView<R>>& o, int* overflow, const R pos[3], R u, R v, R w, R e,
                                      uint32_t hist, uint64_t serial, int type, uint32_t tag) {
-  const int slot = wave_reserve(q.count);
+  const QueueView<R>& q = o.q;
+  const int slot = wave_slot(q.count, o.st);
   if (slot >= q.cap) { atomicAdd(overflow, 1); return; }
   q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
   q.u[slot] = u; q.v[slot] = v; q.w[slot] = w; q.e[slot] = e;
@@ -142,6 +199,12 @@ __device__ __forceinline__ void push(const QueueView<R>& q, int* overflow, const
   q.slo[slot] = (uint32_t)serial;
   q.shi[slot] = (uint32_t)(serial >> 32);
   q.tt[slot] = make_tt(type, 0u, tag);
+}
+
+__device__ __forceinline__ void close_block(const OutQ<QueueView<R>>& o) {
+  const int base = o.st[0], used = o.st[1];
+  const int l = lane_id();
+  if (used < kBlk && l >= used && l < kBlk && base + l < o.q.cap) o.q.tt[base + l] = kEmptySlot;
 }
 
 struct Track {
@@ -170,16 +233,14 @@ __device__ __forceinline__ void load_track(const QueueView<R>& q, int i, const K
   t.rng.ndraw = (tt >> 2) & 63u;
   t.rng.nchild = 0;
   // a stream resumed mid-block (draws taken by the source) needs its block
-  if (t.rng.ndraw & 3u) {
-    uint32_t o[4];
-    Philox::block(t.rng.ndraw >> 2, t.rng.slo, t.rng.shi, t.rng.hist, t.rng.k0, t.rng.k1, o);
-    t.rng.b0 = o[0]; t.rng.b1 = o[1]; t.rng.b2 = o[2]; t.rng.b3 = o[3];
-  }
+  if (t.rng.ndraw & 3u)
+    philox_block(t.rng.ndraw >> 2, t.rng.slo, t.rng.shi, t.rng.hist, t.rng.k0, t.rng.k1, &t.rng.b0, &t.rng.b1,
+                 &t.rng.b2, &t.rng.b3);
 }
 
 // PerformAnnihilation (SimDPMLike.cc:753-791): two back-to-back 0.511 photons
 template <class BlockAcc>
-__device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Track& t, const QueueView<R>& qg) {
+__device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Track& t, const OutQ<QueueView<R>>& qg) {
   count(a, kCntAnnih);
   const R cost = (R)1.0 - (R)2.0 * t.rng.uni<R>();
   const R sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
@@ -257,8 +318,8 @@ __global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S,
 // photons to qg (next round's photon queue)
 // ---------------------------------------------------------------------------
 template <int GEOM, class BlockAcc>
-__device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Track& t, const QueueView<R>& qe,
-                                            const QueueView<R>& qg) {
+__device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Track& t, const OutQ<QueueView<R>>& qe,
+                                            const OutQ<QueueView<R>>& qg) {
   const Tables<R>& T = P.T;
   const R kInvEMC2 = (R)(1.0 / kEMC2);
   count(a, kCntGSteps);
@@ -344,6 +405,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Tr
 // start of a popped track (SimDPMLike.cc:86-95); false if it ends here
 template <int GEOM, class BlockAcc>
 __device__ __forceinline__ bool start_track(BlockAcc& a, const KParams<R>& P, Track& t) {
+  if (t.type == (int)kEmptySlot) return false;
   if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) box_of<R, GEOM>(P.G, t.pos, t.box);
   count(a, kCntVoxelVisits);
   const int imat = material_index<R, GEOM>(P.G, t.box);
@@ -351,41 +413,71 @@ __device__ __forceinline__ bool start_track(BlockAcc& a, const KParams<R>& P, Tr
   return true;
 }
 
+// Input queues are consumed in per-wave ranges of kBlk entries: a wave takes a
+// range from the queue head with one atomic and hands its entries to lanes
+// whose particle has ended (ballot rank), so the head sees one atomic per
+// kBlk particles. Runs with the whole wave converged (loop top).
+struct InRange {
+  int next, end;
+};
+
+template <class Start>
+__device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, int* head, int n_in, Start start) {
+  uint64_t need = __ballot(!has);
+  while (need != 0ull && !exhausted) {
+    if (r.next >= r.end) {
+      int b = 0;
+      if (lane_id() == 0) b = atomicAdd(head, kBlk);
+      b = __shfl(b, 0);
+      if (b >= n_in) { exhausted = true; break; }
+      r.next = b;
+      r.end = min(b + kBlk, n_in);
+    }
+    const int avail = r.end - r.next;
+    const bool mine = ((need >> lane_id()) & 1ull) != 0ull;
+    const int rank = (int)__popcll(need & lanes_below());
+    const bool take = mine && rank < avail;
+    const int idx = r.next + rank;
+    r.next += min(avail, (int)__popcll(need));
+    if (take) has = start(idx);
+    need &= ~__ballot(take);
+  }
+}
+
 template <int GEOM>
 __global__ void __launch_bounds__(kBlock) k_photons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
                                                     QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
+  __shared__ int wout[kBlock / 64][2][2];
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
+    wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
+  }
   acc_init(acc, GEOM == GEOM_SLAB);
+  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
+  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
   const int n_in = *in.count < in.cap ? *in.count : in.cap;
   Track t;
   bool has = false, exhausted = false;
+  InRange rng_in{0, 0};
   for (;;) {
-    if (!exhausted) {
-      const uint64_t need = __ballot(!has);
-      if (need) {
-        const int leader = __ffsll((long long)need) - 1;
-        int base = 0;
-        if (lane_id() == leader) base = atomicAdd(head, __popcll(need));
-        base = __shfl(base, leader);
-        if (base + __popcll(need) >= n_in) exhausted = true;
-        if (!has) {
-          const int idx = base + (int)__popcll(need & lanes_below());
-          if (idx < n_in) {
-            load_track(in, idx, P, t);
-            has = start_track<GEOM>(acc, P, t);
-          }
-        }
-      }
-    }
+    if (!exhausted)
+      refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
+        load_track(in, idx, P, t);
+        return start_track<GEOM>(acc, P, t);
+      });
     if (!__any(has)) {
       if (exhausted) break;
       continue;
     }
     if (has) {
-      has = photon_step<GEOM>(acc, P, t, qe, qg);
+      has = photon_step<GEOM>(acc, P, t, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
   }
+  close_block(oe);
+  close_block(og);
   acc_flush(acc, P, GEOM == GEOM_SLAB);
 }
 
@@ -415,7 +507,7 @@ __device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, 
 
 template <int GEOM, class BlockAcc>
 __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, Track& t, EState& s,
-                                              const QueueView<R>& qe, const QueueView<R>& qg) {
+                                              const OutQ<QueueView<R>>& qe, const OutQ<QueueView<R>>& qg) {
   const Tables<R>& T = P.T;
   // distance to boundary at the start of every step (SimDPMLike.cc:257, 451)
   const R stepGeom = distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
@@ -568,38 +660,37 @@ template <int GEOM>
 __global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
                                                       QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
+  __shared__ int wout[kBlock / 64][2][2];
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
+    wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
+  }
   acc_init(acc, GEOM == GEOM_SLAB);
+  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
+  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
   const int n_in = *in.count < in.cap ? *in.count : in.cap;
   Track t;
   EState s;
   bool has = false, exhausted = false;
+  InRange rng_in{0, 0};
   for (;;) {
-    if (!exhausted) {
-      const uint64_t need = __ballot(!has);
-      if (need) {
-        const int leader = __ffsll((long long)need) - 1;
-        int base = 0;
-        if (lane_id() == leader) base = atomicAdd(head, __popcll(need));
-        base = __shfl(base, leader);
-        if (base + __popcll(need) >= n_in) exhausted = true;
-        if (!has) {
-          const int idx = base + (int)__popcll(need & lanes_below());
-          if (idx < n_in) {
-            load_track(in, idx, P, t);
-            has = electron_init<GEOM>(acc, P, t, s);
-          }
-        }
-      }
-    }
+    if (!exhausted)
+      refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
+        load_track(in, idx, P, t);
+        return electron_init<GEOM>(acc, P, t, s);
+      });
     if (!__any(has)) {
       if (exhausted) break;
       continue;
     }
     if (has) {
-      has = electron_step<GEOM>(acc, P, t, s, qe, qg);
+      has = electron_step<GEOM>(acc, P, t, s, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
   }
+  close_block(oe);
+  close_block(og);
   acc_flush(acc, P, GEOM == GEOM_SLAB);
 }
 
