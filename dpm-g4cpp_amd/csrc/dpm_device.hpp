@@ -156,10 +156,14 @@ __host__ __device__ __forceinline__ uint64_t child_serial(uint64_t parent, uint3
   return mix64(parent ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)));
 }
 
-// one Philox block for (block, serial, history): out of line so that the ~20
-// draw sites of the transport share one copy of the 10 rounds (code size)
-__device__ __noinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
-                                          uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3);
+// one Philox block for (block, serial, history), inlined: a call costs more
+// (argument moves, spills around the call) than the 10 rounds themselves
+__device__ __forceinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
+                                             uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3) {
+  uint32_t o[4];
+  Philox::block(blk, slo, shi, hist, k0, k1, o);
+  *o0 = o[0]; *o1 = o[1]; *o2 = o[2]; *o3 = o[3];
+}
 
 struct Rng {
   uint32_t k0, k1;        // key (seed)

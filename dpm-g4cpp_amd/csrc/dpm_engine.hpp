@@ -8,11 +8,12 @@
 
 namespace dpm {
 
-// counters accumulated by the kernels (the first nine are the byte-model
-// counters exported through dpm_stats.counters)
+// counters accumulated by the kernels (the first ten are exported through
+// dpm_stats.counters; deposits counts scoring calls, tally atomics the FP64
+// atomics actually issued after write-combining)
 enum {
   kCntVoxelVisits = 0, kCntDeposits, kCntSB, kCntMoller, kCntKN, kCntGS, kCntSecondaries, kCntESteps, kCntGSteps,
-  kCntPair, kCntAnnih, kNumCounters
+  kCntTallyAtomics, kCntPair, kCntAnnih, kNumCounters
 };
 
 // counters per kernel kind: 0 photons, 1 electrons

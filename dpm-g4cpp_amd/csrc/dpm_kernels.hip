@@ -28,13 +28,6 @@ using R = DPM_REAL;
 
 constexpr int kBlock = 256;
 
-__device__ __noinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
-                                          uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3) {
-  uint32_t o[4];
-  Philox::block(blk, slo, shi, hist, k0, k1, o);
-  *o0 = o[0]; *o1 = o[1]; *o2 = o[2]; *o3 = o[3];
-}
-
 __device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, double invLogDelta_d, uint32_t k1) {
   const double phd = (log(e) - logMinE_d) * invLogDelta_d;
   const int ie = (int)phd;
@@ -170,9 +163,49 @@ __device__ __forceinline__ void score(BlockAcc& a, const KParams<R>& P, R e, con
     }
   } else {
     const size_t k = ((size_t)box[2] * P.G.ny + box[1]) * P.G.nx + box[0];
+    count(a, kCntTallyAtomics);
     unsafeAtomicAdd(&P.tally[(size_t)tag * P.nvox + k], (double)e);
   }
   if (P.hist_edep) unsafeAtomicAdd(&P.hist_edep[hist - P.hist_base], (double)e);
+}
+
+// Write-combined voxel scoring of the electron kernel: consecutive deposits of
+// a lane into the same voxel are summed in an FP64 register and reach the
+// tally with one atomic when the voxel changes or the particle ends.
+struct WComb {
+  int64_t vox;     // pending voxel (-1: none)
+  double e;        // pending energy
+};
+
+template <int GEOM, class BlockAcc>
+__device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb& w, R e, const int box[3],
+                                         uint32_t tag, uint32_t hist) {
+  if (GEOM == GEOM_SLAB) {
+    score<GEOM>(a, P, e, box, tag, hist);
+    return;
+  }
+  count(a, kCntDeposits);
+  const int64_t k = (int64_t)((size_t)tag * P.nvox + ((size_t)box[2] * P.G.ny + box[1]) * P.G.nx + box[0]);
+  if (k != w.vox) {
+    if (w.vox >= 0) {
+      count(a, kCntTallyAtomics);
+      unsafeAtomicAdd(&P.tally[w.vox], w.e);
+    }
+    w.vox = k;
+    w.e = 0.0;
+  }
+  w.e += (double)e;
+  if (P.hist_edep) unsafeAtomicAdd(&P.hist_edep[hist - P.hist_base], (double)e);
+}
+
+template <class BlockAcc>
+__device__ __forceinline__ void flush_wc(BlockAcc& a, const KParams<R>& P, WComb& w) {
+  if (w.vox >= 0) {
+    count(a, kCntTallyAtomics);
+    unsafeAtomicAdd(&P.tally[w.vox], w.e);
+    w.vox = -1;
+    w.e = 0.0;
+  }
 }
 
 template <class BlockAcc>
@@ -506,7 +539,7 @@ __device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, 
 }
 
 template <int GEOM, class BlockAcc>
-__device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, Track& t, EState& s,
+__device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, Track& t, EState& s, WComb& wc,
                                               const OutQ<QueueView<R>>& qe, const OutQ<QueueView<R>>& qg) {
   const Tables<R>& T = P.T;
   // distance to boundary at the start of every step (SimDPMLike.cc:257, 451)
@@ -569,7 +602,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   t.pos[0] += t.dir[0] * stepLength;
   t.pos[1] += t.dir[1] * stepLength;
   t.pos[2] += t.dir[2] * stepLength;
-  score<GEOM>(a, P, edep, t.box, t.tag, t.rng.hist);
+  score_wc<GEOM>(a, P, wc, edep, t.box, t.tag, t.rng.hist);
   if (what == 0) return true;
   if (what == 4) return false;
   // the interaction switch of Simulate (SimDPMLike.cc:169-244)
@@ -592,7 +625,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
       t.E = t.E - eGamma;
     }
     if (t.E < ecut) {
-      score<GEOM>(a, P, t.E, t.box, t.tag, t.rng.hist);
+      score_wc<GEOM>(a, P, wc, t.E, t.box, t.tag, t.rng.hist);
       t.E = (R)0.0;
       if (t.type == 2) annihilate(a, P, t, qg);
       return false;
@@ -672,6 +705,7 @@ __global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R>
   const int n_in = *in.count < in.cap ? *in.count : in.cap;
   Track t;
   EState s;
+  WComb wc{-1, 0.0};
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
   for (;;) {
@@ -685,8 +719,9 @@ __global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R>
       continue;
     }
     if (has) {
-      has = electron_step<GEOM>(acc, P, t, s, oe, og);
+      has = electron_step<GEOM>(acc, P, t, s, wc, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
+      if (!has) flush_wc(acc, P, wc);
     }
   }
   close_block(oe);

@@ -17,7 +17,8 @@ CLI_PATH = os.path.join(PKG, "bin", "dpm_Simulate")
 DPM_FP32, DPM_FP64 = 0, 1
 ELECTRON, PHOTON, POSITRON = -1, 0, 1
 SRC_PENCIL, SRC_FIELD = 0, 1
-COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries", "e_steps", "g_steps")
+COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries", "e_steps", "g_steps",
+                 "tally_atomics")
 
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
@@ -42,10 +43,10 @@ class RunOptions(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("e_scored", ctypes.c_double), ("e_escaped", ctypes.c_double),
-                ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 9),
+                ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 10),
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
-                ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 9),
-                ("counters_electron", ctypes.c_uint64 * 9)]
+                ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
+                ("counters_electron", ctypes.c_uint64 * 10)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,

@@ -154,7 +154,7 @@ typedef struct dpm_run_options {
 
 enum {
   DPM_CNT_VOXEL_VISITS = 0, DPM_CNT_DEPOSITS, DPM_CNT_SB, DPM_CNT_MOLLER, DPM_CNT_KN, DPM_CNT_GS,
-  DPM_CNT_SECONDARIES, DPM_CNT_E_STEPS, DPM_CNT_G_STEPS, DPM_NCNT
+  DPM_CNT_SECONDARIES, DPM_CNT_E_STEPS, DPM_CNT_G_STEPS, DPM_CNT_TALLY_ATOMICS, DPM_NCNT
 };
 
 typedef struct dpm_stats {
