@@ -28,10 +28,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per event (FP32 transport): voxel visit = u8 material +
-# f32 density; deposit = FP64 read-modify-write; alias sample = 3 entries of
-# 16 B; GS sample = 2 entries of 16 B + a(E) and E-grid pairs; secondary =
-# one 44 B queue record written and read back
-BYTES = dict(voxel_visits=5, deposits=16, sb=48, moller=48, kn=48, gs=48, secondaries=88)
+# f32 density; tally update = FP64 read-modify-write (consecutive deposits of
+# one particle into one voxel are combined in registers first, so this counts
+# the global atomics, not the deposits); alias sample = 3 entries of 16 B; GS
+# sample = 2 entries of 16 B + a(E) and E-grid pairs; secondary = one 44 B
+# queue record written and read back
+BYTES = dict(voxel_visits=5, tally_atomics=16, sb=48, moller=48, kn=48, gs=48, secondaries=88)
 
 
 def algorithmic_bytes(c):
@@ -106,10 +108,15 @@ def main():
     rho = np.ones((nx, nx, nx), dtype=np.float32)
     ph = dpm_amd.Phantom.voxel(t, mat, rho, lbox)
 
+    # result arrays in page-locked host memory, allocated once like any
+    # caller that repeats runs would
+    outs = {prec: (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
+            for prec in (dpm_amd.DPM_FP32, dpm_amd.DPM_FP64)}
+
     def step(precision=dpm_amd.DPM_FP32, seed=1):
         return dpm_amd.simulate(t, ph, nhist, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=seed,
                                 precision=precision, nbatch=args.nbatch, chunk=args.chunk,
-                                waves_per_cu=args.waves_per_cu)
+                                waves_per_cu=args.waves_per_cu, out=outs[precision])
 
     for _ in range(args.warmup):
         step()

@@ -306,7 +306,7 @@ struct Workspace {
 };
 
 // histories per queue fill when the caller does not choose
-constexpr int64_t kDefaultChunk = (int64_t)1 << 22;
+constexpr int64_t kDefaultChunk = (int64_t)1 << 24;
 
 // grow-only device buffer kept by a tables handle across runs
 struct DevBuf {
@@ -345,6 +345,22 @@ __global__ void k_fold(double* rows, int nrows, int64_t nvox, double* sum, doubl
       sum[v] += s;
       sum2[v] += s2;
     }
+  }
+}
+
+// sum of a tally into *out (block partial sums, one atomic per block)
+__global__ void k_sum(const double* x, int64_t n, double* out) {
+  __shared__ double part[4];
+  double acc = 0.0;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    acc += x[v];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b += part[w];
+    unsafeAtomicAdd(out, b);
   }
 }
 
@@ -625,7 +641,8 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
 
 int finish_stats(dpm_tables* t, dpm_stats* st) {
   StatsDev h;
-  HIP_TRY(hipMemcpy(&h, t->stats, sizeof(h), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(&h, t->stats, sizeof(h), hipMemcpyDeviceToHost, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
   if (st) {
     for (int i = 0; i < DPM_NCNT; ++i) {
       st->counters_photon[i] = h.cnt[0][i];
@@ -636,6 +653,7 @@ int finish_stats(dpm_tables* t, dpm_stats* st) {
     st->n_annih = h.cnt[0][kCntAnnih] + h.cnt[1][kCntAnnih];
     st->e_escaped = h.escaped;
     st->e_primary = h.e_primary;
+    st->e_scored = h.scored;
     st->hist_size = std::max(20, h.maxiz + 10);
   }
   return DPM_OK;
@@ -671,6 +689,19 @@ extern "C" {
 
 const char* dpm_last_error(void) { return g_last_error.c_str(); }
 int dpm_abi_version(void) { return DPM_ABI_VERSION; }
+
+int dpm_host_alloc(size_t bytes, void** out) {
+  if (!out || bytes == 0) return set_error(DPM_E_INVALID, "null argument or zero size");
+  *out = nullptr;
+  if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess || !*out)
+    return set_error(DPM_E_NOMEM, "cannot allocate " + std::to_string(bytes) + " bytes of pinned host memory");
+  return DPM_OK;
+}
+
+int dpm_host_free(void* p) {
+  if (p && hipHostFree(p) != hipSuccess) return set_error(DPM_E_DEVICE, "hipHostFree failed");
+  return DPM_OK;
+}
 
 int dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out) {
   if (!out) return set_error(DPM_E_INVALID, "null output handle");
@@ -849,6 +880,12 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   rc = opt.precision == DPM_FP64
            ? run_transport<double>(t, p, S, nhist, &opt, t->wsd, t->td, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local)
            : run_transport<float>(t, p, S, nhist, &opt, t->wsf, t->tf, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local);
+  if (rc == DPM_OK) {
+    // scored energy: reduced on the device from the tally that is returned
+    hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, nbatch > 0 ? d_sum : d_rows, nb,
+                       &t->stats->scored);
+    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
+  }
   if (rc == DPM_OK) rc = finish_stats(t, &local);
   if (rc == DPM_OK) {
     const double* res = nbatch > 0 ? d_sum : d_rows;
@@ -862,11 +899,6 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
     if (rc == DPM_OK && opt.hist_edep &&
         hipMemcpy(opt.hist_edep, d_he, sizeof(double) * nhist, hipMemcpyDeviceToHost) != hipSuccess)
       rc = set_error(DPM_E_DEVICE, "per-history copy failed");
-    if (rc == DPM_OK) {
-      double sc = 0;
-      for (int64_t k = 0; k < nb; ++k) sc += tally[k];
-      local.e_scored = sc;
-    }
   }
   if (st) *st = local;
   return rc;
@@ -941,6 +973,11 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   int rc = opt.precision == DPM_FP64
                ? run_transport<double>(t, p, S, total, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
                : run_transport<float>(t, p, S, total, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
+  if (rc == DPM_OK) {
+    hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, d_rows, (int64_t)nb * p->nvox,
+                       &t->stats->scored);
+    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
+  }
   if (rc == DPM_OK) rc = finish_stats(t, &local);
   if (rc == DPM_OK) {
     hipLaunchKernelGGL(k_rowmax, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax);

@@ -156,27 +156,45 @@ __host__ __device__ __forceinline__ uint64_t child_serial(uint64_t parent, uint3
   return mix64(parent ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)));
 }
 
-// one Philox block for (block, serial, history), inlined: a call costs more
-// (argument moves, spills around the call) than the 10 rounds themselves
-__device__ __forceinline__ void philox_block(uint32_t blk, uint32_t slo, uint32_t shi, uint32_t hist, uint32_t k0,
-                                             uint32_t k1, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3) {
-  uint32_t o[4];
-  Philox::block(blk, slo, shi, hist, k0, k1, o);
-  *o0 = o[0]; *o1 = o[1]; *o2 = o[2]; *o3 = o[3];
-}
-
+// Words of one particle stream, buffered up to two Philox blocks ahead. The
+// transport loops call topup() at wave-uniform points (after a refill and
+// before each step), so the 10 rounds run there for all lanes at once; a draw
+// site only selects a register. A draw that finds the buffer empty (more
+// draws in one step than the buffer held) generates its block in place.
+// Buffering changes when a block is computed, never which words are consumed.
 struct Rng {
   uint32_t k0, k1;        // key (seed)
   uint32_t hist;
   uint32_t slo, shi;      // serial
-  uint32_t ndraw;
+  uint32_t ndraw;         // words consumed so far
   uint32_t nchild;
+  uint32_t nblk;          // buffered blocks: b = block ndraw>>2 if nblk >= 1, c = the next if nblk == 2
   uint32_t b0, b1, b2, b3;
+  uint32_t c0, c1, c2, c3;
+  __device__ __forceinline__ void topup() {
+    while (nblk < 2u) {
+      uint32_t o[4];
+      Philox::block((ndraw >> 2) + nblk, slo, shi, hist, k0, k1, o);
+      if (nblk == 0u) { b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3]; }
+      else { c0 = o[0]; c1 = o[1]; c2 = o[2]; c3 = o[3]; }
+      ++nblk;
+    }
+  }
   __device__ __forceinline__ uint32_t draw24() {
+    if (nblk == 0u) {
+      uint32_t o[4];
+      Philox::block(ndraw >> 2, slo, shi, hist, k0, k1, o);
+      b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
+      nblk = 1u;
+    }
     const uint32_t j = ndraw & 3u;
-    if (j == 0u) philox_block(ndraw >> 2, slo, shi, hist, k0, k1, &b0, &b1, &b2, &b3);
+    // the empty asm keeps the words in registers: without it the select chain
+    // is folded into an indexed load, which moves the whole track to scratch
+    uint32_t x0 = b0, x1 = b1, x2 = b2, x3 = b3;
+    asm("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3));
+    const uint32_t w = j == 0u ? x0 : (j == 1u ? x1 : (j == 2u ? x2 : x3));
     ++ndraw;
-    const uint32_t w = j == 0u ? b0 : (j == 1u ? b1 : (j == 2u ? b2 : b3));
+    if (j == 3u) { b0 = c0; b1 = c1; b2 = c2; b3 = c3; --nblk; }
     return w >> 8;
   }
   template <typename R> __device__ __forceinline__ R uni() { return (R)draw24() * (R)(1.0 / 16777216.0); }

@@ -21,6 +21,7 @@ struct StatsDev {
   unsigned long long cnt[2][kNumCounters];
   double escaped;
   double e_primary;
+  double scored;            // sum of the returned tally (set after the run)
   int maxiz;
   int pad;
 };

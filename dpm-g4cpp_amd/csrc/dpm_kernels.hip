@@ -28,6 +28,14 @@ using R = DPM_REAL;
 
 constexpr int kBlock = 256;
 
+// FP32 photon kernel: held to 128 VGPRs (4 waves per SIMD) without spills;
+// the FP64 build would spill under the same bound
+#if DPM_PREC == 32
+#define DPM_PHOTON_WAVES __attribute__((amdgpu_waves_per_eu(4)))
+#else
+#define DPM_PHOTON_WAVES
+#endif
+
 __device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, double invLogDelta_d, uint32_t k1) {
   const double phd = (log(e) - logMinE_d) * invLogDelta_d;
   const int ie = (int)phd;
@@ -265,10 +273,9 @@ __device__ __forceinline__ void load_track(const QueueView<R>& q, int i, const K
   t.rng.shi = q.shi[i];
   t.rng.ndraw = (tt >> 2) & 63u;
   t.rng.nchild = 0;
-  // a stream resumed mid-block (draws taken by the source) needs its block
-  if (t.rng.ndraw & 3u)
-    philox_block(t.rng.ndraw >> 2, t.rng.slo, t.rng.shi, t.rng.hist, t.rng.k0, t.rng.k1, &t.rng.b0, &t.rng.b1,
-                 &t.rng.b2, &t.rng.b3);
+  // nothing buffered: the first top-up (or draw) computes block ndraw>>2, so a
+  // stream resumed mid-block (draws taken by the source) continues in place
+  t.rng.nblk = 0;
 }
 
 // PerformAnnihilation (SimDPMLike.cc:753-791): two back-to-back 0.511 photons
@@ -298,7 +305,7 @@ __global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S,
   Rng rng;
   rng.k0 = P.k0; rng.k1 = P.k1;
   rng.hist = (uint32_t)hid;
-  rng.slo = 0; rng.shi = 0; rng.ndraw = 0; rng.nchild = 0;
+  rng.slo = 0; rng.shi = 0; rng.ndraw = 0; rng.nchild = 0; rng.nblk = 0;
   int bl = 0;
   double x0 = S.x0, y0 = S.y0, e = S.energy;
   int particle = S.particle;
@@ -478,7 +485,7 @@ __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, i
 }
 
 template <int GEOM>
-__global__ void __launch_bounds__(kBlock) k_photons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
+__global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES k_photons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
                                                     QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
@@ -498,6 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_photons(KParams<R> P, QueueView<R> i
     if (!exhausted)
       refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
         load_track(in, idx, P, t);
+        t.rng.topup();
         return start_track<GEOM>(acc, P, t);
       });
     if (!__any(has)) {
@@ -505,6 +513,7 @@ __global__ void __launch_bounds__(kBlock) k_photons(KParams<R> P, QueueView<R> i
       continue;
     }
     if (has) {
+      t.rng.topup();
       has = photon_step<GEOM>(acc, P, t, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
@@ -712,6 +721,7 @@ __global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R>
     if (!exhausted)
       refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
         load_track(in, idx, P, t);
+        t.rng.topup();
         return electron_init<GEOM>(acc, P, t, s);
       });
     if (!__any(has)) {
@@ -719,6 +729,7 @@ __global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R>
       continue;
     }
     if (has) {
+      t.rng.topup();
       has = electron_step<GEOM>(acc, P, t, s, wc, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
       if (!has) flush_wc(acc, P, wc);

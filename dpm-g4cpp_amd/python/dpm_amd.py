@@ -22,7 +22,8 @@ COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "second
 
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
-            "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_last_error", "dpm_abi_version")
+            "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_last_error",
+            "dpm_abi_version")
 
 
 class DpmError(RuntimeError):
@@ -90,6 +91,8 @@ def lib():
         L.dpm_beamlets_fetch.argtypes = [vp, vp, vp]
         L.dpm_sample.argtypes = [vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
         L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
+        L.dpm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.dpm_host_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -180,16 +183,51 @@ class Phantom:
             pass
 
 
+class _Pinned:
+    """Owner of one dpm_host_alloc block; frees it when the last array view goes."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().dpm_host_alloc(nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.dpm_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(n, dtype=np.float64):
+    """A 1-D numpy array of n elements in page-locked host memory (fast tally copies)."""
+    dtype = np.dtype(dtype)
+    owner = _Pinned(max(1, n * dtype.itemsize))
+    buf = (ctypes.c_char * owner.nbytes).from_address(owner.ptr)
+    buf._dpm_owner = owner
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
-             seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False):
-    """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None)."""
+             seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
+             out=None):
+    """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
+
+    out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
+    phantom.nbins elements to write into (e.g. from pinned_empty), returned
+    reshaped to the phantom's shape."""
     src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
                  x0=x0, y0=y0, field_x=field, field_y=field)
     he = np.zeros(nhist) if per_history else None
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
                      waves_per_cu=waves_per_cu, hist_edep=_ptr(he))
-    tally = np.zeros(phantom.nbins)
-    steps = np.zeros(phantom.nbins)
+    if out is not None:
+        tally, steps = out
+        for a in (tally, steps):
+            if a.dtype != np.float64 or a.size != phantom.nbins or not a.flags["C_CONTIGUOUS"]:
+                raise ValueError("out arrays must be contiguous float64 with %d elements" % phantom.nbins)
+    else:
+        tally = np.zeros(phantom.nbins)
+        steps = np.zeros(phantom.nbins)
     st = Stats()
     _check(lib().dpm_simulate(tables.h, phantom.h, ctypes.byref(src), nhist, ctypes.byref(opt), _ptr(tally),
                               _ptr(steps), ctypes.byref(st)))

@@ -219,6 +219,13 @@ int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* ener
  *         5 max total IMFP(E) 6 total(E,m) 7 Compton(E,m) 8 pair(E,m) */
 int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energy, const int* imat, double* out);
 
+/* Page-locked host memory for result arrays (no counterpart in the
+ * reference, whose results go to a file). Tally copies into such memory run
+ * at the link's DMA rate; into pageable memory they are staged by the
+ * runtime. Any host pointer is accepted by the entry points above. */
+int dpm_host_alloc(size_t bytes, void** out);
+int dpm_host_free(void* p);
+
 const char* dpm_last_error(void);
 int dpm_abi_version(void);
 
