@@ -211,7 +211,7 @@ int build_pack(const dpm_table_desc* d, Tables<R>& T, void** dev) {
   const dpm_gs_desc& g = d->gs;
   std::vector<GSEntry<R>> ge((size_t)g.ne * g.ns);
   for (size_t k = 0; k < ge.size(); ++k) {
-    ge[k].u = (R)g.u[k]; ge[k].a = (R)g.a[k]; ge[k].b = (R)g.b[k]; ge[k].pad = (R)0;
+    ge[k].u = (R)g.u[k]; ge[k].a = (R)g.a[k]; ge[k].b = (R)g.b[k]; ge[k].v = (R)(1.0 - g.u[k]);
   }
   std::vector<R> gen(g.ne), gtr(g.ne);
   for (int i = 0; i < g.ne; ++i) { gen[i] = (R)g.energy[i]; gtr[i] = (R)g.transf[i]; }

@@ -77,7 +77,10 @@ struct AliasSet {               // SimSBTables / SimMollerTables / SimKNTables
 };
 
 template <typename R>
-struct alignas(sizeof(R) * 4) GSEntry { R u, a, b, pad; };
+struct alignas(sizeof(R) * 4) GSEntry {
+  R u, a, b;
+  R v;                          // 1 - u, rounded from FP64 (FP32 inversion near u = 1)
+};
 
 template <typename R>
 struct GSSet {                  // SimGSTables
@@ -135,11 +138,12 @@ struct Philox {
                                                uint32_t k1, uint32_t out[4]) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-      const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-      const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-      const uint32_t n0 = hi1 ^ c1 ^ k0;
-      const uint32_t n2 = hi0 ^ c3 ^ k1;
-      c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+      // full 64-bit products: one v_mad_u64_u32 each instead of mul_hi + mul_lo
+      const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+      const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+      const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+      const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+      c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
       k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
@@ -264,6 +268,13 @@ __device__ __forceinline__ R alias_sample(const AliasEntry<R>* tab, int ns, uint
   return xv + xd * rsqrt_(r2);
 }
 
+// exp(xi * (ln E - ln Emin)) of the SB and Moller energy transfers in FP32:
+// the argument (up to ~6) is formed in FP64 from the FP32 ln E, so that its
+// rounding does not add ~5e-7 relative error to the result
+__device__ __forceinline__ float transfer_exp(float xi, float lpe, double logMinE_d) {
+  return rexp((float)((double)xi * ((double)lpe - logMinE_d)));
+}
+
 // SimSBTables::SampleEnergyTransfer (SimSBTables.cc:93-114): photon energy
 template <typename R>
 __device__ __forceinline__ R sample_sb(const AliasSet<R>& s, R e, int imat, uint32_t k1, uint32_t k2, uint32_t k3,
@@ -277,7 +288,7 @@ __device__ __forceinline__ R sample_sb(const AliasSet<R>& s, R e, int imat, uint
     const R kappa = kappac * rexp(-xi * (s.logMinE - lpe));
     return kappa * e;
   }
-  return s.minE * rexp(xi * (lpe - s.logMinE));
+  return s.minE * transfer_exp((float)xi, (float)lpe, s.logMinE_d);
 }
 
 // SimMollerTables::SampleEnergyTransfer (SimMollerTables.cc:69-89): secondary energy
@@ -288,6 +299,7 @@ __device__ __forceinline__ R sample_moller(const AliasSet<R>& s, R e, uint32_t k
   const int ie = pick_energy_bin(lpe, s.logMinE, s.invLogDelta, s.logMinE_d, s.invLogDelta_d, e, k1, (int*)nullptr);
   if (ebin) *ebin = ie;
   const R xi = alias_sample(s.e + (size_t)ie * s.ns, s.ns, k2, k3, abin);
+  if (sizeof(R) == 4) return transfer_exp((float)xi, (float)lpe, s.logMinE_d) * (R)0.5 * s.minE;
   const R dum1 = lpe - s.logMinE;
   return rexp(xi * dum1) * (R)0.5 * s.minE;
 }
@@ -301,37 +313,69 @@ __device__ __forceinline__ R sample_kn(const AliasSet<R>& s, R e, uint32_t k1, u
   if (ebin) *ebin = ie;
   const R xi = alias_sample(s.e + (size_t)ie * s.ns, s.ns, k2, k3, abin);
   const R kappa = e / (R)kEMC2;
+  if (sizeof(R) == 4) return rexp((R)((double)rlog((R)1. + (R)2. * kappa) * ((double)xi - 1.0)));
   return rexp(rlog((R)1. + (R)2. * kappa) * (xi - (R)1.));
 }
 
+// Rational inverse of SimGSTables::SampleAngularDeflection (SimGSTables.cc:
+// 96-119) for the picked energy table `ie` (and lower grid index `ielow` of
+// the transform parameter). Returns 1-cos(theta) (exact form 2aU/(1-U+a)).
+template <typename R>
+__device__ __forceinline__ R gs_rational(const GSSet<R>& s, int ie, int ielow, R e, uint32_t k2, int* cbin) {
+  // cumulative bin (int)(r2/deltaCum) == floor(r2*(ns-1)) exactly on the lattice
+  const uint32_t rest = k2 * (uint32_t)(s.ns - 1);
+  const int il = (int)(rest >> 24);
+  if (cbin) *cbin = il;
+  const GSEntry<R> g0 = s.e[ie * s.ns + il];
+  const GSEntry<R> g1 = s.e[ie * s.ns + il + 1];
+  const R a0 = s.transf[ielow], a1 = s.transf[ielow + 1];
+  const R e0 = s.egrid[ielow], e1 = s.egrid[ielow + 1];
+  if (sizeof(R) == 8) {
+    const R r2 = (R)k2 * (R)(1.0 / 16777216.0);
+    const R resid = r2 - (R)il * s.deltaCum;
+    const R dum1 = ((R)1.0 + g0.a + g0.b) * s.deltaCum * resid;
+    const R dum2 = s.deltaCum * s.deltaCum + g0.a * s.deltaCum * resid + g0.b * resid * resid;
+    const R theU = g0.u + dum1 / dum2 * (g1.u - g0.u);
+    const R parTransf = (a1 - a0) / (e1 - e0) * (e - e0) + a0;
+    return (R)2. * parTransf * theU / ((R)1. - theU + parTransf);
+  }
+  // FP32 tables, arithmetic after the loads in FP64 (full rate on this GPU).
+  // 1-U is interpolated from the stored complements instead of formed as
+  // 1 - U, which cancels as U -> 1, and the fraction within the cumulative
+  // bin is kept in FP64: in the last bins (u1 - u0 ~ 0.03, a ~ 1e-3) an
+  // FP32 fraction alone moves cos(theta) by several 1e-6.
+  const double dc = (double)s.deltaCum;
+  const double resid = (double)(rest & 0xFFFFFFu) * (1.0 / 16777216.0) * dc;
+  const double ga = (double)g0.a, gb = (double)g0.b;
+  const double f = (1.0 + ga + gb) * dc * resid / (dc * dc + ga * dc * resid + gb * resid * resid);
+  const double U = (double)g0.u + f * ((double)g1.u - (double)g0.u);
+  const double omU = (double)g0.v + f * ((double)g1.v - (double)g0.v);
+  const double pa = ((double)a1 - (double)a0) / ((double)e1 - (double)e0) * ((double)e - (double)e0) + (double)a0;
+  return (R)(2.0 * pa * U / (omU + pa));
+}
+
 // SimGSTables::SampleAngularDeflection (SimGSTables.cc:84-119). Returns
-// 1-cos(theta) (exact form 2aU/(1-U+a)); cos(theta) = 1 - result.
+// 1-cos(theta); cos(theta) = 1 - result.
 template <typename R>
 __device__ __forceinline__ R sample_gs_omu(const GSSet<R>& s, R e, uint32_t k1, uint32_t k2, int* ebin, int* cbin) {
   const R lpe = rlog(e);
   int ielow = 0;
   const int ie = pick_energy_bin(lpe, s.logMinE, s.invLogDelta, s.logMinE_d, s.invLogDelta_d, e, k1, &ielow);
   if (ebin) *ebin = ie;
-  // cumulative bin (int)(r2/deltaCum) == floor(r2*(ns-1)) exactly on the lattice
-  const uint32_t rest = k2 * (uint32_t)(s.ns - 1);
-  const int il = (int)(rest >> 24);
-  if (cbin) *cbin = il;
-  R resid;
-  if (sizeof(R) == 8) {
-    const R r2 = (R)k2 * (R)(1.0 / 16777216.0);
-    resid = r2 - (R)il * s.deltaCum;
-  } else {
-    resid = (R)(rest & 0xFFFFFFu) * (R)(1.0 / 16777216.0) * s.deltaCum;
-  }
-  const GSEntry<R> g0 = s.e[ie * s.ns + il];
-  const R u1 = s.e[ie * s.ns + il + 1].u;
-  const R dum1 = ((R)1.0 + g0.a + g0.b) * s.deltaCum * resid;
-  const R dum2 = s.deltaCum * s.deltaCum + g0.a * s.deltaCum * resid + g0.b * resid * resid;
-  const R theU = g0.u + dum1 / dum2 * (u1 - g0.u);
-  const R a0 = s.transf[ielow], a1 = s.transf[ielow + 1];
-  const R e0 = s.egrid[ielow], e1 = s.egrid[ielow + 1];
-  const R parTransf = (a1 - a0) / (e1 - e0) * (e - e0) + a0;
-  return (R)2. * parTransf * theU / ((R)1. - theU + parTransf);
+  return gs_rational(s, ie, ielow, e, k2, cbin);
+}
+
+// per-lane choice between two spline tables (one evaluation site for both)
+template <typename R>
+__device__ __forceinline__ SplineTab<R> pick_spline(bool c, const SplineTab<R>& a, const SplineTab<R>& b) {
+  SplineTab<R> r;
+  r.node = c ? a.node : b.node;
+  r.n = c ? a.n : b.n;
+  r.logMinX = c ? a.logMinX : b.logMinX;
+  r.invLogDelta = c ? a.invLogDelta : b.invLogDelta;
+  r.emin = c ? a.emin : b.emin;
+  r.emaxm = c ? a.emaxm : b.emaxm;
+  return r;
 }
 
 // RotateToLabFrame (SimDPMLike.cc:638-656)

@@ -28,12 +28,14 @@ using R = DPM_REAL;
 
 constexpr int kBlock = 256;
 
-// FP32 photon kernel: held to 128 VGPRs (4 waves per SIMD) without spills;
-// the FP64 build would spill under the same bound
+// FP32 kernels: photons held to 128 VGPRs (4 waves per SIMD), electrons to
+// 168 (3 waves) without spills; the FP64 build would spill under such bounds
 #if DPM_PREC == 32
 #define DPM_PHOTON_WAVES __attribute__((amdgpu_waves_per_eu(4)))
+#define DPM_ELECTRON_WAVES __attribute__((amdgpu_waves_per_eu(3)))
 #else
 #define DPM_PHOTON_WAVES
+#define DPM_ELECTRON_WAVES
 #endif
 
 __device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, double invLogDelta_d, uint32_t k1) {
@@ -614,92 +616,113 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   score_wc<GEOM>(a, P, wc, edep, t.box, t.tag, t.rng.hist);
   if (what == 0) return true;
   if (what == 4) return false;
-  // the interaction switch of Simulate (SimDPMLike.cc:169-244)
-  if (what == 1) {
-    if (t.E > T.gcut) {
-      // PerformBrem (SimDPMLike.cc:661-695)
-      count(a, kCntSB);
-      const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24(), k3 = t.rng.draw24();
-      const R eGamma = sample_sb(T.sb, t.E, imat, k1, k2, k3, (int*)nullptr, (int*)nullptr);
-      const R dum0 = (R)(kEMC2 * 0.7071067812) / (t.E + (R)kEMC2);
-      const R cost = rmax((R)-1.0, rmin((R)1.0, (R)1.0 - dum0 * dum0));
-      const R sint = rsqrt_(((R)1.0 + cost) * ((R)1.0 - cost));
-      const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
-      R sp, cp;
-      rsincos(phi, &sp, &cp);
-      R du = sint * cp, dv = sint * sp, dw = cost;
-      rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
-      count(a, kCntSecondaries);
-      push(qg, P.overflow, t.pos, du, dv, dw, eGamma, t.rng.hist, t.rng.next_child(), 0, t.tag);
-      t.E = t.E - eGamma;
-    }
-    if (t.E < ecut) {
-      score_wc<GEOM>(a, P, wc, t.E, t.box, t.tag, t.rng.hist);
-      t.E = (R)0.0;
-      if (t.type == 2) annihilate(a, P, t, qg);
-      return false;
-    }
-    s.nBrem = -t.rng.lnu<R>();
-    return true;
-  }
-  if (what == 2) {
-    if (t.E > (R)2. * ecut) {
-      // PerformMoller (SimDPMLike.cc:699-728)
-      count(a, kCntMoller);
-      const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24(), k3 = t.rng.draw24();
-      const R secEkin = sample_moller(T.mo, t.E, k1, k2, k3, (int*)nullptr, (int*)nullptr);
-      const R k2EMC2 = (R)(2.0 * kEMC2);
-      const R cost = rsqrt_(secEkin * (t.E + k2EMC2) / (t.E * (secEkin + k2EMC2)));
-      const R secCost = rmin((R)1.0, cost);
-      const R sint = rsqrt_(((R)1.0 + secCost) * ((R)1.0 - secCost));
-      const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
-      R sp, cp;
-      rsincos(phi, &sp, &cp);
-      R du = sint * cp, dv = sint * sp, dw = secCost;
-      rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
-      count(a, kCntSecondaries);
-      push(qe, P.overflow, t.pos, du, dv, dw, secEkin, t.rng.hist, t.rng.next_child(), 1, t.tag);
-      t.E -= secEkin;
-    }
-    s.nMol = -t.rng.lnu<R>();
-    s.invMol = spline_value(T.moller, t.E, 0);
-    return true;
-  }
-  // what == 3: MSC hinge or end of the MSC step
-  if (s.hinge) {
-    // PerformMSCAngularDeflection (SimDPMLike.cc:731-750)
-    count(a, kCntGS);
-    const uint32_t k1 = t.rng.draw24(), k2 = t.rng.draw24();
-    const R omu = sample_gs_omu(T.gs, s.E0, k1, k2, (int*)nullptr, (int*)nullptr);
-    R cost, sint;
-    if (sizeof(R) == 8) {
-      cost = rmax((R)-1.0, rmin((R)1.0, (R)1. - omu));
-      sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
+  // The interaction switch of Simulate (SimDPMLike.cc:169-244). PerformBrem
+  // (:661-695), PerformMoller (:699-728) and PerformMSCAngularDeflection
+  // (:731-750) share one pass through the log-energy table pick, one alias
+  // or GS inversion, the azimuth, the rotation to the lab frame and the #mfp
+  // re-sample, so a wave whose lanes took different interactions runs each
+  // part once. Every lane still draws its own uniforms in the reference's
+  // order: sampler (3 for SB/Moller, 2 for GS), azimuth, then re-sample.
+  const bool brem = what == 1 && t.E > T.gcut;
+  const bool mol = what == 2 && t.E > (R)2. * ecut;
+  const bool hinge = what == 3 && s.hinge;
+  if (brem || mol || hinge) {
+    const uint32_t k1 = t.rng.draw24();
+    const uint32_t k2 = t.rng.draw24();
+    const uint32_t k3 = hinge ? 0u : t.rng.draw24();
+    const R es = hinge ? s.E0 : t.E;  // GS is sampled at the pre-step energy (:220)
+    const R lpe = rlog(es);
+    const R logMinE = brem ? T.sb.logMinE : (mol ? T.mo.logMinE : T.gs.logMinE);
+    const R invLogDelta = brem ? T.sb.invLogDelta : (mol ? T.mo.invLogDelta : T.gs.invLogDelta);
+    const double logMinE_d = brem ? T.sb.logMinE_d : (mol ? T.mo.logMinE_d : T.gs.logMinE_d);
+    const double invLogDelta_d = brem ? T.sb.invLogDelta_d : (mol ? T.mo.invLogDelta_d : T.gs.invLogDelta_d);
+    int ielow = 0;
+    const int ie = pick_energy_bin(lpe, logMinE, invLogDelta, logMinE_d, invLogDelta_d, es, k1, &ielow);
+    R cost, q, esec = (R)0;
+    if (hinge) {
+      const R omu = gs_rational(T.gs, ie, ielow, es, k2, (int*)nullptr);
+      if (sizeof(R) == 8) {
+        cost = rmax((R)-1.0, rmin((R)1.0, (R)1. - omu));
+        q = ((R)1.0 - cost) * ((R)1.0 + cost);
+      } else {
+        const R o = rmax((R)0.0, rmin((R)2.0, omu));
+        cost = (R)1.0 - o;
+        q = o * ((R)2.0 - o);
+      }
     } else {
-      const R o = rmax((R)0.0, rmin((R)2.0, omu));
-      cost = (R)1.0 - o;
-      sint = rsqrt_(o * ((R)2.0 - o));
+      // SimSBTables.cc:93-114 (photon energy) or SimMollerTables.cc:69-89
+      const AliasEntry<R>* tab = brem ? T.sb.e + (size_t)(imat * T.sb.ne + ie) * T.sb.ns : T.mo.e + (size_t)ie * T.mo.ns;
+      const int ns = brem ? T.sb.ns : T.mo.ns;
+      const R minE = brem ? T.sb.minE : T.mo.minE;
+      const R xi = alias_sample(tab, ns, k2, k3, (int*)nullptr);
+      if (sizeof(R) == 8) {
+        const R ex = rexp(brem ? -xi * (logMinE - lpe) : xi * (lpe - logMinE));
+        esec = brem ? minE / es * ex * es : ex * (R)0.5 * minE;
+      } else {
+        const R ex = transfer_exp((float)xi, (float)lpe, logMinE_d);
+        esec = brem ? minE * ex : ex * (R)0.5 * minE;
+      }
+      if (brem) {
+        // photon angle of the DPM mean-angle approximation (:673-684)
+        const R dum0 = (R)(kEMC2 * 0.7071067812) / (t.E + (R)kEMC2);
+        cost = rmax((R)-1.0, rmin((R)1.0, (R)1.0 - dum0 * dum0));
+      } else {
+        const R k2EMC2 = (R)(2.0 * kEMC2);
+        cost = rmin((R)1.0, rsqrt_(esec * (t.E + k2EMC2) / (t.E * (esec + k2EMC2))));
+      }
+      q = ((R)1.0 + cost) * ((R)1.0 - cost);
     }
+    const R sint = rsqrt_(q);
     const R phi = (R)(2.0 * kPI) * t.rng.uni<R>();
     R sp, cp;
     rsincos(phi, &sp, &cp);
-    R u1 = sint * cp, u2 = sint * sp, u3 = cost;
-    rotate_to_lab(u1, u2, u3, t.dir[0], t.dir[1], t.dir[2]);
-    t.dir[0] = u1; t.dir[1] = u2; t.dir[2] = u3;
-    s.nTr1 = s.nTr10;
-    s.hinge = false;
-  } else {
-    s.E0 = t.E;
-    s.nTr1 = spline_value(T.k1, s.E0, 0);
-    s.nTr10 = t.rng.uni<R>() * s.nTr1;
-    s.nTr1 -= s.nTr10;
-    s.hinge = true;
+    R du = sint * cp, dv = sint * sp, dw = cost;
+    rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
+    if (hinge) {
+      count(a, kCntGS);
+      t.dir[0] = du; t.dir[1] = dv; t.dir[2] = dw;
+      s.nTr1 = s.nTr10;
+      s.hinge = false;
+    } else {
+      if (brem) count(a, kCntSB);  // count() needs one counter per call site
+      else count(a, kCntMoller);
+      count(a, kCntSecondaries);
+      if (brem) push(qg, P.overflow, t.pos, du, dv, dw, esec, t.rng.hist, t.rng.next_child(), 0, t.tag);
+      else push(qe, P.overflow, t.pos, du, dv, dw, esec, t.rng.hist, t.rng.next_child(), 1, t.tag);
+      t.E = t.E - esec;
+    }
+  }
+  if (what == 1 && t.E < ecut) {
+    // below the cut after the brem: deposit the rest (:179-188)
+    score_wc<GEOM>(a, P, wc, t.E, t.box, t.tag, t.rng.hist);
+    t.E = (R)0.0;
+    if (t.type == 2) annihilate(a, P, t, qg);
+    return false;
+  }
+  if (what == 1 || what == 2) {
+    const R n = -t.rng.lnu<R>();
+    if (what == 1) s.nBrem = n;
+    else s.nMol = n;
+  }
+  // Moller IMFP after a Moller step, or K1 at the end of an MSC step: one
+  // spline evaluation at the current energy (both tables are material-free)
+  if (what == 2 || (what == 3 && !hinge)) {
+    const R v = spline_value(pick_spline(what == 2, T.moller, T.k1), t.E, 0);
+    if (what == 2) {
+      s.invMol = v;
+    } else {
+      s.E0 = t.E;
+      s.nTr1 = v;
+      s.nTr10 = t.rng.uni<R>() * s.nTr1;
+      s.nTr1 -= s.nTr10;
+      s.hinge = true;
+    }
   }
   return true;
 }
 
 template <int GEOM>
-__global__ void __launch_bounds__(kBlock) k_electrons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
+__global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES k_electrons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
                                                       QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];

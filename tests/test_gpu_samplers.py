@@ -71,6 +71,30 @@ def test_fp32_samplers_match_reference(gpu_tables3, tables3, gold, kind):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", KINDS)
+def test_fp32_samplers_random_uniforms(gpu_tables3, tables3, gold, kind):
+    """20000 random (energy, uniforms) per sampler over the fixture's energy
+    range against the FP64 oracle at the same float energy: bins identical,
+    values within 1e-6 (relative; absolute for cos(theta) of GS)."""
+    rng = np.random.default_rng(2024 + KINDS.index(kind))
+    e_fix = gold[kind + "_e"]
+    n = 20000
+    e = np.exp(rng.uniform(np.log(e_fix.min()), np.log(e_fix.max()), n)).astype(np.float32).astype(np.float64)
+    k = rng.integers(0, 1 << 24, size=(n, 3), dtype=np.uint32)
+    imat = rng.integers(0, 3, n).astype(np.int32) if kind == "sb" else None
+    out, eb, ab = gpu_tables3.sample(kind, e, k[:, 0], k[:, 1], None if kind == "gs" else k[:, 2], imat=imat,
+                                     precision=dpm_amd.DPM_FP32)
+    oracle_lib.load(tables3)
+    ref = np.empty(n)
+    for i in range(n):
+        v, reb, rab = oracle_lib.sample(kind, float(e[i]), k[i] / float(1 << 24), int(imat[i]) if kind == "sb" else 0)
+        assert eb[i] == reb and ab[i] == rab, (kind, i, e[i], eb[i], reb, ab[i], rab)
+        ref[i] = v
+    err = np.abs(out - ref) if kind == "gs" else np.abs(out - ref) / np.abs(ref)
+    assert err.max() <= 1e-6, (kind, err.max(), int(np.argmax(err)))
+
+
+@pytest.mark.gpu
 def test_eval_matches_reference(gpu_tables3, gold):
     rows = gold["ev_rows"]
     ref = gold["ev_ref"]
