@@ -433,6 +433,8 @@ struct dpm_tables {
   hipEvent_t ev[10] = {};
   // tally buffers reused across runs
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
+  DevBuf buf_src;            // source description of the current run
+  DevBuf buf_blaux;          // beamlet row maxima, counts and row pointers
   // CSR of the last beamlet run
   int32_t* csr_col = nullptr;
   double* csr_val = nullptr;
@@ -481,34 +483,115 @@ template <typename R>
 struct Launch;
 template <>
 struct Launch<float> {
-  static hipError_t source(const KParams<float>& P, const SourceParams& S, uint64_t h0, int n,
-                           const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
-    return launch_source_f32(P, S, h0, n, qe, qg, st);
+  static hipError_t photons(const KParams<float>& P, const SourceParams* S, const PrimRange& r, int g, int b,
+                            const QueueView<float>& in, int* h, const QueueView<float>& qe,
+                            const QueueView<float>& qg, hipStream_t st) {
+    return launch_photons_f32(P, S, r, g, b, in, h, qe, qg, st);
   }
-  static hipError_t photons(const KParams<float>& P, int g, int b, const QueueView<float>& in, int* h,
-                            const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
-    return launch_photons_f32(P, g, b, in, h, qe, qg, st);
-  }
-  static hipError_t electrons(const KParams<float>& P, int g, int b, const QueueView<float>& in, int* h,
-                              const QueueView<float>& qe, const QueueView<float>& qg, hipStream_t st) {
-    return launch_electrons_f32(P, g, b, in, h, qe, qg, st);
+  static hipError_t electrons(const KParams<float>& P, const SourceParams* S, const PrimRange& r, int g, int b,
+                              const QueueView<float>& in, int* h, const QueueView<float>& qe,
+                              const QueueView<float>& qg, hipStream_t st) {
+    return launch_electrons_f32(P, S, r, g, b, in, h, qe, qg, st);
   }
 };
 template <>
 struct Launch<double> {
-  static hipError_t source(const KParams<double>& P, const SourceParams& S, uint64_t h0, int n,
-                           const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
-    return launch_source_f64(P, S, h0, n, qe, qg, st);
+  static hipError_t photons(const KParams<double>& P, const SourceParams* S, const PrimRange& r, int g, int b,
+                            const QueueView<double>& in, int* h, const QueueView<double>& qe,
+                            const QueueView<double>& qg, hipStream_t st) {
+    return launch_photons_f64(P, S, r, g, b, in, h, qe, qg, st);
   }
-  static hipError_t photons(const KParams<double>& P, int g, int b, const QueueView<double>& in, int* h,
-                            const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
-    return launch_photons_f64(P, g, b, in, h, qe, qg, st);
-  }
-  static hipError_t electrons(const KParams<double>& P, int g, int b, const QueueView<double>& in, int* h,
-                              const QueueView<double>& qe, const QueueView<double>& qg, hipStream_t st) {
-    return launch_electrons_f64(P, g, b, in, h, qe, qg, st);
+  static hipError_t electrons(const KParams<double>& P, const SourceParams* S, const PrimRange& r, int g, int b,
+                              const QueueView<double>& in, int* h, const QueueView<double>& qe,
+                              const QueueView<double>& qg, hipStream_t st) {
+    return launch_electrons_f64(P, S, r, g, b, in, h, qe, qg, st);
   }
 };
+
+// Source of one call as the host describes it: entries (plain source: one;
+// beamlets: one per beamlet) split by particle kind, in call order
+struct HostEntry {
+  uint64_t first;   // global id of the entry's first history
+  uint64_t n;       // histories
+  double x, y, e;
+  int particle;
+  int row;          // tally row, or -1 (batch row / row 0)
+};
+struct HostSource {
+  int kind = DPM_SRC_PENCIL;
+  double emin = 0, emax = 0, fx = 0, fy = 0;
+  std::vector<HostEntry> list[2];   // [0] photons, [1] e-/e+
+  void add(const HostEntry& h) { list[h.particle == DPM_PHOTON ? 0 : 1].push_back(h); }
+  uint64_t count(int k) const {
+    uint64_t c = 0;
+    for (const HostEntry& h : list[k]) c += h.n;
+    return c;
+  }
+};
+
+// Uploads the source into one device block: SourceParams followed by the
+// per-list arrays (cum, first, x, y, e, particle, row); returns the device
+// address of the SourceParams.
+int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, uint64_t hist_begin, uint64_t nhist,
+                  SourceParams** out) {
+  size_t bytes = (sizeof(SourceParams) + 255) & ~(size_t)255;
+  size_t off[2][7];
+  for (int k = 0; k < 2; ++k) {
+    const size_t n = hs.list[k].size();
+    const size_t sz[7] = {8 * (n + 1), 8 * n, 8 * n, 8 * n, 8 * n, 4 * n, 4 * n};
+    for (int a = 0; a < 7; ++a) {
+      off[k][a] = bytes;
+      bytes += (sz[a] + 255) & ~(size_t)255;
+    }
+  }
+  if (!t->buf_src.ensure(bytes)) return set_error(DPM_E_NOMEM, "cannot allocate the source description");
+  std::vector<unsigned char> h(bytes, 0);
+  char* base = (char*)t->buf_src.p;
+  SourceParams S;
+  std::memset(&S, 0, sizeof(S));
+  S.kind = hs.kind;
+  S.nbatch = nbatch;
+  S.emin = hs.emin;
+  S.emax = hs.emax;
+  S.fx = hs.fx;
+  S.fy = hs.fy;
+  S.z0 = z0;
+  S.hist_begin = hist_begin;
+  S.nhist_total = nhist;
+  for (int k = 0; k < 2; ++k) {
+    const std::vector<HostEntry>& L = hs.list[k];
+    const int n = (int)L.size();
+    uint64_t* cum = (uint64_t*)(h.data() + off[k][0]);
+    uint64_t* first = (uint64_t*)(h.data() + off[k][1]);
+    double* x = (double*)(h.data() + off[k][2]);
+    double* y = (double*)(h.data() + off[k][3]);
+    double* e = (double*)(h.data() + off[k][4]);
+    int* pt = (int*)(h.data() + off[k][5]);
+    int* row = (int*)(h.data() + off[k][6]);
+    cum[0] = 0;
+    for (int i = 0; i < n; ++i) {
+      cum[i + 1] = cum[i] + L[i].n;
+      first[i] = L[i].first;
+      x[i] = L[i].x; y[i] = L[i].y; e[i] = L[i].e;
+      pt[i] = L[i].particle;
+      row[i] = L[i].row;
+    }
+    PrimList& D = S.list[k];
+    D.n = n;
+    D.cum = (const uint64_t*)(base + off[k][0]);
+    D.first = (const uint64_t*)(base + off[k][1]);
+    D.x = (const double*)(base + off[k][2]);
+    D.y = (const double*)(base + off[k][3]);
+    D.e = (const double*)(base + off[k][4]);
+    D.particle = (const int*)(base + off[k][5]);
+    D.row = (const int*)(base + off[k][6]);
+  }
+  std::memcpy(h.data(), &S, sizeof(S));
+  // synchronous: the host staging vector goes out of scope on return
+  HIP_TRY(hipMemcpy(t->buf_src.p, h.data(), bytes, hipMemcpyHostToDevice));
+  *out = (SourceParams*)t->buf_src.p;
+  return DPM_OK;
+}
 
 // boundaries of batch b: first history ceil(b*N/nb)
 uint64_t batch_start(uint64_t b, uint64_t n, uint64_t nb) {
@@ -516,10 +599,12 @@ uint64_t batch_start(uint64_t b, uint64_t n, uint64_t nb) {
   return (uint64_t)((num + nb - 1) / nb);
 }
 
-// Runs histories [0, nhist) of the source (offset by opt->hist_begin) and
-// leaves the tally on the device in `rows` (rows x nbins|nvox doubles).
+// Runs every history of the source and leaves the tally on the device in
+// `rows` (rows x nbins|nvox doubles). Histories are taken in chunks over the
+// concatenation [photon list | e-/e+ list]; the first round of a chunk
+// generates its primaries inside the transport kernels.
 template <typename R>
-int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t nhist, const dpm_run_options* opt,
+int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, const dpm_run_options* opt,
                   Workspace<R>& ws, Tables<R>& T, double* d_rows, int nrows_per_chunk, double* d_steps,
                   double* d_sum, double* d_sum2, double* d_hist_edep, dpm_stats* st) {
   const hipStream_t s = t->stream;
@@ -539,7 +624,12 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
   P.stats = t->stats;
   P.overflow = t->ctl + 5;
 
-  const int nbatch = (S.nbeamlet == 0 && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
+  const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
+  const uint64_t nhist = n_ph + n_el;
+  // batch rows need one source entry (history index = history id - begin)
+  const size_t nent = hs.list[0].size() + hs.list[1].size();
+  const bool single = nent == 1 && (hs.list[0].empty() ? hs.list[1][0] : hs.list[0][0]).row < 0;
+  const int nbatch = (single && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
   int64_t chunk = opt->chunk > 0 ? opt->chunk : kDefaultChunk;
   if (chunk > ((int64_t)1 << 24)) chunk = (int64_t)1 << 24;
   chunk = std::min<int64_t>(chunk, (int64_t)nhist);
@@ -551,17 +641,17 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
   const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + holes + 65536, (int64_t)1 << 30);
   int rc = ws.ensure(cap, t->ctl);
   if (rc != DPM_OK) return rc;
-  S.hist_begin = opt->hist_begin;
-  S.nhist_total = nhist;
-  S.nbatch = nbatch;
-  S.z0 = p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox;
+  SourceParams* dS = nullptr;
+  rc = upload_source(t, hs, p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox, nbatch,
+                     opt->hist_begin, nhist, &dS);
+  if (rc != DPM_OK) return rc;
 
   HIP_TRY(hipEventRecord(t->ev[0], s));
   uint64_t done = 0;
   uint64_t next_batch = 0;
   int rounds = 0;
-  float ms_src = 0, ms_ph = 0, ms_el = 0, ms_fold = 0;
-  int n_src = 0, n_ph = 0, n_el = 0, n_fold = 0;
+  float ms_ph = 0, ms_el = 0, ms_fold = 0;
+  int c_ph = 0, c_el = 0, c_fold = 0;
   while (done < nhist) {
     uint64_t n = std::min<uint64_t>((uint64_t)chunk, nhist - done);
     uint32_t bfirst = 0;
@@ -578,13 +668,13 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
       nrows = (int)(b1 - b0);
       next_batch = b1;
     }
-    S.batch_first = bfirst;
+    // this chunk's primaries in each kind's index space
+    const uint64_t a = done, b = done + n;
+    const PrimRange r_ph{std::min(a, n_ph), std::min(b, n_ph), bfirst};
+    const PrimRange r_el{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst};
+    const PrimRange none{0, 0, bfirst};
     HIP_TRY(hipMemsetAsync(t->ctl, 0, 8 * sizeof(int), s));
     int cur = 0;
-    HIP_TRY(hipEventRecord(t->ev[1], s));
-    HIP_TRY(Launch<R>::source(P, S, opt->hist_begin + done, (int)n, ws.q[2 + cur], ws.q[0 + cur], s));
-    HIP_TRY(hipEventRecord(t->ev[2], s));
-    ++n_src;
     bool first = true;
     for (;;) {
       const int nxt = cur ^ 1;
@@ -594,20 +684,22 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
       HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
       HIP_TRY(hipEventRecord(t->ev[3], s));
       P.kind = 0;
-      HIP_TRY(Launch<R>::photons(P, p->geom, blocks, ws.q[0 + cur], t->ctl + 4, ws.q[2 + cur], ws.q[0 + nxt], s));
+      HIP_TRY(Launch<R>::photons(P, dS, first ? r_ph : none, p->geom, blocks, ws.q[0 + cur], t->ctl + 4,
+                                 ws.q[2 + cur], ws.q[0 + nxt], s));
       HIP_TRY(hipEventRecord(t->ev[4], s));
       HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
       HIP_TRY(hipEventRecord(t->ev[5], s));
       P.kind = 1;
-      HIP_TRY(Launch<R>::electrons(P, p->geom, blocks, ws.q[2 + cur], t->ctl + 4, ws.q[2 + nxt], ws.q[0 + nxt], s));
+      HIP_TRY(Launch<R>::electrons(P, dS, first ? r_el : none, p->geom, blocks, ws.q[2 + cur], t->ctl + 4,
+                                   ws.q[2 + nxt], ws.q[0 + nxt], s));
       HIP_TRY(hipEventRecord(t->ev[6], s));
       int counts[6];
       HIP_TRY(hipMemcpyAsync(counts, t->ctl, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
-      if (first) { ms_src += elapsed(t->ev[1], t->ev[2]); first = false; }
+      first = false;
       ms_ph += elapsed(t->ev[3], t->ev[4]);
       ms_el += elapsed(t->ev[5], t->ev[6]);
-      ++n_ph; ++n_el;
+      ++c_ph; ++c_el;
       ++rounds;
       if (counts[5] != 0)
         return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
@@ -624,7 +716,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
       HIP_TRY(hipEventRecord(t->ev[8], s));
       HIP_TRY(hipEventSynchronize(t->ev[8]));
       ms_fold += elapsed(t->ev[7], t->ev[8]);
-      ++n_fold;
+      ++c_fold;
     }
     done += n;
   }
@@ -633,8 +725,9 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, SourceParams S, uint64_t 
   if (st) {
     st->seconds = elapsed(t->ev[0], t->ev[9]) * 1e-3;
     st->rounds = rounds;
-    st->kernel_ms[0] = ms_src; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
-    st->kernel_calls[0] = n_src; st->kernel_calls[1] = n_ph; st->kernel_calls[2] = n_el; st->kernel_calls[3] = n_fold;
+    // slot 0 (source) stays 0: primaries are generated inside the transport kernels
+    st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
+    st->kernel_calls[0] = 0; st->kernel_calls[1] = c_ph; st->kernel_calls[2] = c_el; st->kernel_calls[3] = c_fold;
   }
   return DPM_OK;
 }
@@ -659,17 +752,16 @@ int finish_stats(dpm_tables* t, dpm_stats* st) {
   return DPM_OK;
 }
 
-void source_from(const dpm_source* src, SourceParams& S) {
-  std::memset(&S, 0, sizeof(S));
-  S.kind = src->kind;
-  S.particle = src->particle;
-  S.energy = src->energy;
-  S.emin = src->emin;
-  S.emax = src->emax;
-  S.x0 = src->x0;
-  S.y0 = src->y0;
-  S.fx = src->field_x;
-  S.fy = src->field_y;
+// the plain source: one entry of nhist histories from id hist_begin
+HostSource source_from(const dpm_source* src, uint64_t nhist, uint64_t hist_begin) {
+  HostSource hs;
+  hs.kind = src->kind;
+  hs.emin = src->emin;
+  hs.emax = src->emax;
+  hs.fx = src->field_x;
+  hs.fy = src->field_y;
+  hs.add(HostEntry{hist_begin, nhist, src->x0, src->y0, src->energy, src->particle, -1});
+  return hs;
 }
 
 int check_source(const dpm_tables* t, int particle, double e, double emin, double emax) {
@@ -758,7 +850,7 @@ void dpm_tables_destroy(dpm_tables* t) {
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   t->wsf.release();
   t->wsd.release();
-  for (DevBuf* b : {&t->buf_rows, &t->buf_steps, &t->buf_sum, &t->buf_sum2, &t->buf_he}) b->release();
+  for (DevBuf* b : {&t->buf_rows, &t->buf_steps, &t->buf_sum, &t->buf_sum2, &t->buf_he, &t->buf_src, &t->buf_blaux}) b->release();
   if (t->csr_col) (void)hipFree(t->csr_col);
   if (t->csr_val) (void)hipFree(t->csr_val);
   if (t->dev_f) (void)hipFree(t->dev_f);
@@ -847,8 +939,7 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   if (rc != DPM_OK) return rc;
   HIP_TRY(hipSetDevice(t->device));
   const hipStream_t s = t->stream;
-  SourceParams S;
-  source_from(src, S);
+  const HostSource hs = source_from(src, nhist, opt.hist_begin);
 
   const bool slab = p->geom == GEOM_SLAB;
   const int nbatch = (!slab && opt.nbatch > 1) ? opt.nbatch : 0;
@@ -878,8 +969,8 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   dpm_stats local;
   std::memset(&local, 0, sizeof(local));
   rc = opt.precision == DPM_FP64
-           ? run_transport<double>(t, p, S, nhist, &opt, t->wsd, t->td, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local)
-           : run_transport<float>(t, p, S, nhist, &opt, t->wsf, t->tf, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local);
+           ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local)
+           : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local);
   if (rc == DPM_OK) {
     // scored energy: reduced on the device from the tally that is returned
     hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, nbatch > 0 ? d_sum : d_rows, nb,
@@ -913,66 +1004,43 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   dpm_run_options opt;
   std::memset(&opt, 0, sizeof(opt));
   if (opt_in) opt = *opt_in;
+  if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  // one source entry per beamlet; its histories keep the ids they have when
+  // the beamlets run one after the other from hist_begin
+  HostSource hs;
   uint64_t total = 0;
-  std::vector<uint64_t> off(nb);
-  std::vector<double> bx(nb), by(nb), be(nb);
-  std::vector<int> bt(nb);
   for (int i = 0; i < nb; ++i) {
     int rc = check_source(t, b[i].particle, b[i].energy, 0, 0);
     if (rc != DPM_OK) return rc;
     if (b[i].nhist == 0) return set_error(DPM_E_INVALID, "beamlet without histories");
-    off[i] = opt.hist_begin + total;
+    hs.add(HostEntry{opt.hist_begin + total, b[i].nhist, b[i].x0, b[i].y0, b[i].energy, b[i].particle, i});
     total += b[i].nhist;
-    bx[i] = b[i].x0; by[i] = b[i].y0; be[i] = b[i].energy; bt[i] = b[i].particle;
   }
   if (opt.hist_begin + total > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
   HIP_TRY(hipSetDevice(t->device));
   const hipStream_t s = t->stream;
-  const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
-  double* d_rows = nullptr;
-  void* d_bl = nullptr;
-  double* d_rmax = nullptr;
-  int64_t *d_cnt = nullptr, *d_rowptr = nullptr;
   if (t->csr_col) (void)hipFree(t->csr_col);
   if (t->csr_val) (void)hipFree(t->csr_val);
   t->csr_col = nullptr;
   t->csr_val = nullptr;
   t->csr_nnz = 0;
-  auto cleanup = [&]() {
-    for (void* q : {(void*)d_rows, d_bl, (void*)d_rmax, (void*)d_cnt, (void*)d_rowptr})
-      if (q) (void)hipFree(q);
-  };
-  const size_t blbytes = (size_t)nb * (sizeof(uint64_t) + 3 * sizeof(double) + sizeof(int)) + 1024;
-  if (hipMalloc(&d_rows, rows_bytes) != hipSuccess || hipMalloc(&d_bl, blbytes) != hipSuccess ||
-      hipMalloc(&d_rmax, sizeof(double) * nb) != hipSuccess || hipMalloc(&d_cnt, sizeof(int64_t) * nb) != hipSuccess ||
-      hipMalloc(&d_rowptr, sizeof(int64_t) * (nb + 1)) != hipSuccess) {
-    cleanup();
+  // dense per-beamlet rows (nb x nvox FP64), cached like the other tallies
+  const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
+  if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * (nb + 1) + 1024))
     return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string(rows_bytes >> 20) + " MiB)");
-  }
-  unsigned char* q = (unsigned char*)d_bl;
-  uint64_t* d_off = (uint64_t*)q; q += ((sizeof(uint64_t) * nb + 255) & ~(size_t)255);
-  double* d_bx = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
-  double* d_by = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
-  double* d_be = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
-  int* d_bt = (int*)q;
-  (void)hipMemcpy(d_off, off.data(), sizeof(uint64_t) * nb, hipMemcpyHostToDevice);
-  (void)hipMemcpy(d_bx, bx.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
-  (void)hipMemcpy(d_by, by.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
-  (void)hipMemcpy(d_be, be.data(), sizeof(double) * nb, hipMemcpyHostToDevice);
-  (void)hipMemcpy(d_bt, bt.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+  double* d_rows = t->buf_rows.d();
+  unsigned char* q = (unsigned char*)t->buf_blaux.p;
+  double* d_rmax = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
+  int64_t* d_cnt = (int64_t*)q; q += ((sizeof(int64_t) * (nb + 1) + 255) & ~(size_t)255);
+  int64_t* d_rowptr = (int64_t*)q;
   (void)hipMemsetAsync(d_rows, 0, rows_bytes, s);
   (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), s);
-  SourceParams S;
-  std::memset(&S, 0, sizeof(S));
-  S.kind = DPM_SRC_PENCIL;
-  S.nbeamlet = nb;
-  S.bl_offset = d_off; S.bl_x = d_bx; S.bl_y = d_by; S.bl_e = d_be; S.bl_type = d_bt;
   dpm_stats local;
   std::memset(&local, 0, sizeof(local));
   opt.nbatch = 0;
   int rc = opt.precision == DPM_FP64
-               ? run_transport<double>(t, p, S, total, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
-               : run_transport<float>(t, p, S, total, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
+               ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
+               : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
   if (rc == DPM_OK) {
     hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, d_rows, (int64_t)nb * p->nvox,
                        &t->stats->scored);
@@ -984,8 +1052,9 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
     hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr, d_cnt,
                        (int32_t*)nullptr, (double*)nullptr, 0);
     std::vector<int64_t> cnt(nb);
-    // the run stream is non-blocking: wait for the count pass before the copy
-    if (hipStreamSynchronize(s) != hipSuccess || hipMemcpy(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+    // the run stream is non-blocking: copies are ordered on it
+    if (hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
       rc = set_error(DPM_E_DEVICE, "CSR count failed");
     if (rc == DPM_OK) {
       row_ptr[0] = 0;
@@ -995,8 +1064,10 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
         if (hipMalloc(&t->csr_col, sizeof(int32_t) * nnz) != hipSuccess ||
             hipMalloc(&t->csr_val, sizeof(double) * nnz) != hipSuccess) {
           rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
+        } else if (hipMemcpyAsync(d_rowptr, row_ptr, sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s) !=
+                   hipSuccess) {
+          rc = set_error(DPM_E_DEVICE, "CSR row pointer copy failed");
         } else {
-          (void)hipMemcpy(d_rowptr, row_ptr, sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice);
           hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr,
                              d_cnt, t->csr_col, t->csr_val, 1);
           if (hipStreamSynchronize(s) != hipSuccess) rc = set_error(DPM_E_DEVICE, "CSR build failed");
@@ -1005,7 +1076,6 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
       }
     }
   }
-  cleanup();
   if (st) *st = local;
   return rc;
 }

@@ -52,29 +52,45 @@ struct KParams {
   int kind;                 // 0 photon kernel, 1 electron kernel (counter slot)
 };
 
+// Primaries of one particle kind, enumerated in list order: entry i owns
+// this kind's history indices [cum[i], cum[i+1]), whose global history ids
+// (Philox stream ids) start at first[i]. The plain source is one entry; the
+// beamlet form has one entry per beamlet of the kind.
+struct PrimList {
+  int n;
+  const uint64_t* cum;      // [n + 1]
+  const uint64_t* first;    // [n]
+  const double* x;          // [n] entry point (mm)
+  const double* y;
+  const double* e;          // [n] energy (MeV) unless a spectrum is sampled
+  const int* particle;      // [n] -1 e-, 0 gamma, +1 e+
+  const int* row;           // [n] tally row, or -1: batch row (or row 0)
+};
+
+// Source of one run, resident in device memory for its duration
 struct SourceParams {
-  int kind;                 // 0 pencil, 1 rectangular field
-  int particle;             // -1 e-, 0 gamma, +1 e+
-  double energy, emin, emax;
-  double x0, y0, fx, fy, z0;
-  int nbatch;               // >1: tally row = batch index - batch_first
-  uint32_t batch_first;
+  int kind;                 // 0 pencil, 1 rectangular field around the entry point
+  int nbatch;               // > 1: tally row = batch index - first batch of the chunk
+  double emin, emax;        // emax > emin: energy uniform in [emin, emax)
+  double fx, fy, z0;
   uint64_t hist_begin, nhist_total;
-  int nbeamlet;             // >0: beamlet source, tally row = beamlet
-  const uint64_t* bl_offset;  // [nbeamlet] first global history of each beamlet
-  const double* bl_x;
-  const double* bl_y;
-  const double* bl_e;
-  const int* bl_type;
+  PrimList list[2];         // [0] photons, [1] electrons and positrons
+};
+
+// primaries a transport launch generates after its input queue: this kind's
+// indices [beg, end)
+struct PrimRange {
+  uint64_t beg, end;
+  uint32_t batch_first;
 };
 
 #define DPM_DECLARE_LAUNCHERS(R, SFX)                                                                              \
-  hipError_t launch_source##SFX(const KParams<R>& P, const SourceParams& S, uint64_t h0, int n,                     \
-                                const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st);                   \
-  hipError_t launch_photons##SFX(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in, int* head,     \
-                                 const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st);                  \
-  hipError_t launch_electrons##SFX(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in, int* head,   \
-                                   const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st);                \
+  hipError_t launch_photons##SFX(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int geom,       \
+                                 int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,            \
+                                 const QueueView<R>& qg, hipStream_t st);                                          \
+  hipError_t launch_electrons##SFX(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int geom,     \
+                                   int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,          \
+                                   const QueueView<R>& qg, hipStream_t st);                                        \
   hipError_t launch_sample##SFX(const Tables<R>& T, int kind, int n, const double* e, const int* imat,             \
                                 const uint32_t* r1, const uint32_t* r2, const uint32_t* r3, double* out,           \
                                 int* ebin, int* abin, hipStream_t st);                                             \

@@ -6,12 +6,13 @@
 //
 // Work decomposition (replaces the history loop and LIFO TrackStack of
 // Simulate, SimDPMLike.cc:57-247, TrackStack.hh:17-59):
-//   - particles live in SoA queues (one for photons, one for e-/e+);
-//   - k_source fills the queues with the primaries of a chunk of histories;
+//   - secondaries live in SoA queues (one for photons, one for e-/e+);
 //   - k_photons / k_electrons are persistent kernels: each wave keeps 64
-//     particles in flight and refills finished lanes from the input queue with
-//     one wave-aggregated atomic; secondaries are appended to the output
-//     queues with ballot + prefix-count compaction (one atomic per wave);
+//     particles in flight and refills finished lanes from the input queue,
+//     then from the primaries of the current chunk (generated in registers),
+//     in per-wave ranges of 64 taken with one atomic; secondaries are
+//     appended to the output queues in per-wave blocks with ballot + prefix
+//     counts;
 //   - the host alternates photon/electron rounds until both queues drain.
 // Every particle carries its own Philox stream (seed, history, serial), so
 // results do not depend on the processing order.
@@ -52,17 +53,6 @@ constexpr uint32_t kMaxIter = 200000;  // per-particle iteration cap (terminatio
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
-
-// reserve one slot per active lane of the (possibly divergent) wave with a
-// single atomic; returns this lane's slot
-__device__ __forceinline__ int wave_reserve(int* counter) {
-  const uint64_t act = __ballot(1);
-  const int leader = __ffsll((long long)act) - 1;
-  int base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, __popcll(act));
-  base = __shfl(base, leader);
-  return base + (int)__popcll(act & lanes_below());
-}
 
 // Output queues are appended in per-wave blocks of kBlk consecutive slots: a
 // wave takes a new block from the queue counter only when its current block
@@ -108,6 +98,7 @@ __device__ __forceinline__ int wave_slot(int* counter, volatile int* st) {
 template <int NS>
 struct BlockAccT {
   double escaped;
+  double eprim;                 // energy of the primaries generated by the block
   unsigned cnt[kNumCounters];
   int maxiz;
   double slab_e[NS];
@@ -118,7 +109,7 @@ template <int GEOM> struct AccOf { using type = BlockAccT<(GEOM == GEOM_SLAB) ? 
 template <class BlockAcc>
 __device__ __forceinline__ void acc_init(BlockAcc& a, bool slab) {
   for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x) a.cnt[i] = 0u;
-  if (threadIdx.x == 0) { a.escaped = 0.0; a.maxiz = -1; }
+  if (threadIdx.x == 0) { a.escaped = 0.0; a.eprim = 0.0; a.maxiz = -1; }
   if (slab)
     for (int i = threadIdx.x; i < (int)(sizeof(a.slab_n) / sizeof(unsigned)); i += blockDim.x) {
       a.slab_e[i] = 0.0;
@@ -134,6 +125,7 @@ __device__ __forceinline__ void acc_flush(BlockAcc& a, const KParams<R>& P, bool
     if (a.cnt[i]) atomicAdd((unsigned long long*)&P.stats->cnt[P.kind][i], (unsigned long long)a.cnt[i]);
   if (threadIdx.x == 0) {
     if (a.escaped != 0.0) unsafeAtomicAdd(&P.stats->escaped, a.escaped);
+    if (a.eprim != 0.0) unsafeAtomicAdd(&P.stats->e_primary, a.eprim);
     if (a.maxiz >= 0) atomicMax(&P.stats->maxiz, a.maxiz);
   }
   if (slab) {
@@ -297,61 +289,57 @@ __device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Tra
 }
 
 // ---------------------------------------------------------------------------
-// source: primaries of histories [h0, h0+n) (SimDPMLike.cc:64-75, extended)
+// primaries (SimDPMLike.cc:64-75, extended by fields, spectra and beamlets),
+// generated by the transport kernels themselves when their input queue is
+// drained: history index v of a kind's list -> entry by binary search, then
+// the source draws (field position x, y; spectrum energy) from the history's
+// own stream, in that order
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_source(KParams<R> P, SourceParams S, uint64_t h0, int n,
-                                                   QueueView<R> qe, QueueView<R> qg) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  const uint64_t hid = h0 + (uint64_t)(valid ? i : 0);
-  Rng rng;
-  rng.k0 = P.k0; rng.k1 = P.k1;
-  rng.hist = (uint32_t)hid;
-  rng.slo = 0; rng.shi = 0; rng.ndraw = 0; rng.nchild = 0; rng.nblk = 0;
-  int bl = 0;
-  double x0 = S.x0, y0 = S.y0, e = S.energy;
-  int particle = S.particle;
-  if (S.nbeamlet > 0) {
-    // beamlet of this history: last offset <= hid (offsets are ascending)
-    int lo = 0, hi = S.nbeamlet - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (S.bl_offset[mid] <= hid) lo = mid; else hi = mid - 1;
-    }
-    bl = lo;
-    x0 = S.bl_x[bl]; y0 = S.bl_y[bl]; e = S.bl_e[bl]; particle = S.bl_type[bl];
+__device__ __forceinline__ void gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
+                                            const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
+                                            double& eprim) {
+  int lo = 0, hi = L.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.cum[mid] <= v) lo = mid;
+    else hi = mid - 1;
   }
-  R pos[3] = {(R)x0, (R)y0, (R)S.z0};
-  if (S.kind == 1) {
-    const R ux = rng.uni<R>(), uy = rng.uni<R>();
-    pos[0] = (R)x0 + (ux - (R)0.5) * (R)S.fx;
-    pos[1] = (R)y0 + (uy - (R)0.5) * (R)S.fy;
+  const uint64_t hid = L.first[lo] + (v - L.cum[lo]);
+  t.rng.k0 = P.k0; t.rng.k1 = P.k1;
+  t.rng.hist = (uint32_t)hid;
+  t.rng.slo = 0; t.rng.shi = 0;
+  t.rng.ndraw = 0; t.rng.nchild = 0; t.rng.nblk = 0;
+  t.rng.topup();
+  const double x0 = L.x[lo], y0 = L.y[lo];
+  t.pos[0] = (R)x0;
+  t.pos[1] = (R)y0;
+  t.pos[2] = (R)S->z0;
+  if (S->kind == 1) {
+    const R ux = t.rng.uni<R>(), uy = t.rng.uni<R>();
+    t.pos[0] = (R)x0 + (ux - (R)0.5) * (R)S->fx;
+    t.pos[1] = (R)y0 + (uy - (R)0.5) * (R)S->fy;
   }
-  R E = (R)e;
-  if (S.emax > S.emin) E = (R)S.emin + rng.uni<R>() * (R)(S.emax - S.emin);
-  // sum of primary energies: one atomic per wave (all lanes take part)
-  double ew = valid ? (double)E : 0.0;
-  for (int o = 32; o > 0; o >>= 1) ew += __shfl_xor(ew, o);
-  if (lane_id() == 0 && ew != 0.0) unsafeAtomicAdd(&P.stats->e_primary, ew);
-  if (!valid) return;
-  uint32_t tag = 0;
-  if (S.nbeamlet > 0) tag = (uint32_t)bl;
-  else if (S.nbatch > 1) {
-    const uint64_t rel = hid - S.hist_begin;
-    tag = (uint32_t)((rel * (uint64_t)S.nbatch) / S.nhist_total) - S.batch_first;
-  }
-  const int type = particle == 0 ? 0 : (particle < 0 ? 1 : 2);
-  const QueueView<R>& q = particle == 0 ? qg : qe;
-  int slot;
-  if (particle == 0) slot = wave_reserve(qg.count);
-  else slot = wave_reserve(qe.count);
-  if (slot >= q.cap) { atomicAdd(P.overflow, 1); return; }
-  q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
-  q.u[slot] = (R)0.0; q.v[slot] = (R)0.0; q.w[slot] = (R)1.0;
-  q.e[slot] = E;
-  q.hist[slot] = (uint32_t)hid;
-  q.slo[slot] = 0; q.shi[slot] = 0;
-  q.tt[slot] = make_tt(type, rng.ndraw, tag);
+  R E = (R)L.e[lo];
+  if (S->emax > S->emin) E = (R)S->emin + t.rng.uni<R>() * (R)(S->emax - S->emin);
+  eprim += (double)E;
+  int row = L.row[lo];
+  if (row < 0)
+    row = S->nbatch > 1
+              ? (int)((uint32_t)(((hid - S->hist_begin) * (uint64_t)S->nbatch) / S->nhist_total) - batch_first)
+              : 0;
+  const int pt = L.particle[lo];
+  t.type = pt == 0 ? 0 : (pt < 0 ? 1 : 2);
+  t.tag = (uint32_t)row;
+  t.iter = 0;
+  t.dir[0] = (R)0; t.dir[1] = (R)0; t.dir[2] = (R)1;
+  t.E = E;
+}
+
+// wave sum of the lanes' primary energies into the block accumulator
+template <class BlockAcc>
+__device__ __forceinline__ void flush_eprim(BlockAcc& a, double eprim) {
+  for (int o = 32; o > 0; o >>= 1) eprim += __shfl_xor(eprim, o);
+  if (lane_id() == 0 && eprim != 0.0) atomicAdd(&a.eprim, eprim);
 }
 
 // ---------------------------------------------------------------------------
@@ -487,8 +475,9 @@ __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, i
 }
 
 template <int GEOM>
-__global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES k_photons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
-                                                    QueueView<R> qg) {
+__global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES
+k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head, QueueView<R> qe,
+          QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
   const int wave = threadIdx.x >> 6;
@@ -500,13 +489,16 @@ __global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES k_photons(KParams<R> 
   const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
   const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
   const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const int n_all = n_in + (int)(pr.end - pr.beg);
   Track t;
+  double eprim = 0.0;
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
   for (;;) {
     if (!exhausted)
-      refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
-        load_track(in, idx, P, t);
+      refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
+        if (idx < n_in) load_track(in, idx, P, t);
+        else gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);
         t.rng.topup();
         return start_track<GEOM>(acc, P, t);
       });
@@ -520,6 +512,7 @@ __global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES k_photons(KParams<R> 
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
   }
+  flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
   acc_flush(acc, P, GEOM == GEOM_SLAB);
@@ -722,8 +715,9 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
 }
 
 template <int GEOM>
-__global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES k_electrons(KParams<R> P, QueueView<R> in, int* head, QueueView<R> qe,
-                                                      QueueView<R> qg) {
+__global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES
+k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head,
+            QueueView<R> qe, QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
   const int wave = threadIdx.x >> 6;
@@ -735,15 +729,18 @@ __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES k_electrons(KParams
   const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
   const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
   const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const int n_all = n_in + (int)(pr.end - pr.beg);
   Track t;
   EState s;
   WComb wc{-1, 0.0};
+  double eprim = 0.0;
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
   for (;;) {
     if (!exhausted)
-      refill(has, exhausted, rng_in, head, n_in, [&](int idx) {
-        load_track(in, idx, P, t);
+      refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
+        if (idx < n_in) load_track(in, idx, P, t);
+        else gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);
         t.rng.topup();
         return electron_init<GEOM>(acc, P, t, s);
       });
@@ -758,6 +755,7 @@ __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES k_electrons(KParams
       if (!has) flush_wc(acc, P, wc);
     }
   }
+  flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
   acc_flush(acc, P, GEOM == GEOM_SLAB);
@@ -814,30 +812,23 @@ __global__ void k_eval(Tables<R> T, int which, int n, const double* e, const int
 #define DPM_SFX _f64
 #endif
 
-hipError_t DPM_CAT(launch_source, DPM_SFX)(const KParams<R>& P, const SourceParams& S, uint64_t h0, int n,
-                                           const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_source, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, S, h0, n, qe, qg);
+hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int geom,
+                                            int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,
+                                            const QueueView<R>& qg, hipStream_t st) {
+  if (geom == GEOM_SLAB)
+    hipLaunchKernelGGL(k_photons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+  else
+    hipLaunchKernelGGL(k_photons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   return hipGetLastError();
 }
 
-hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in,
-                                            int* head, const QueueView<R>& qe, const QueueView<R>& qg,
-                                            hipStream_t st) {
+hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, const SourceParams* S, const PrimRange& pr,
+                                              int geom, int blocks, const QueueView<R>& in, int* head,
+                                              const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st) {
   if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL(k_photons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+    hipLaunchKernelGGL(k_electrons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   else
-    hipLaunchKernelGGL(k_photons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
-  return hipGetLastError();
-}
-
-hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, int geom, int blocks, const QueueView<R>& in,
-                                              int* head, const QueueView<R>& qe, const QueueView<R>& qg,
-                                              hipStream_t st) {
-  if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL(k_electrons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
-  else
-    hipLaunchKernelGGL(k_electrons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, in, head, qe, qg);
+    hipLaunchKernelGGL(k_electrons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   return hipGetLastError();
 }
 
