@@ -65,6 +65,47 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
                       "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
 
 
+def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4):
+    """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
+    per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
+    same history ids; only GPU calls are timed (no host thresholding for the
+    plain form, which favours it)."""
+    import numpy as np
+    import phantoms
+    bl = phantoms.beamlets(n_beamlets)
+    total = sum(b[4] for b in bl)
+    # warm-up: same beamlets with few histories (allocates the per-beamlet rows)
+    dpm_amd.simulate_beamlets(t, ph, [b[:4] + (100,) for b in bl], rel_threshold=1e-3, seed=seed)
+    t0 = time.perf_counter()
+    row_ptr, col, val, st = dpm_amd.simulate_beamlets(t, ph, bl, rel_threshold=1e-3, seed=seed)
+    t_batched = time.perf_counter() - t0
+    out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
+    offs = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
+    t0 = time.perf_counter()
+    for i, (x, y, p, e, n) in enumerate(bl):
+        dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed, hist_begin=int(offs[i]), out=out)
+    t_plain = time.perf_counter() - t0
+    # correctness: batched FP32 rows of the largest beamlets vs FP64 plain runs
+    order = np.argsort([-b[4] for b in bl])[:n_check]
+    rates = []
+    for i in order:
+        x, y, p, e, n = bl[i]
+        d64, _, _, _ = dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed,
+                                        hist_begin=int(offs[i]), precision=dpm_amd.DPM_FP64)
+        row = np.zeros(ph.nbins)
+        row[col[row_ptr[i]:row_ptr[i + 1]]] = val[row_ptr[i]:row_ptr[i + 1]]
+        rates.append(gamma.gamma_pass_rate_3d(d64, row.reshape(d64.shape), spacing_mm=lbox, dd=0.01, dta_mm=1.0,
+                                              cutoff=0.1))
+    return {"workload": "config4: %d beamlets on the 150^3 water phantom (e- 6-18 MeV / gamma U(1,20) MeV pencils "
+                        "on a 2 mm grid over 160x160 mm2, lognormal history counts), FP32 transport, CSR of voxels "
+                        "> 0.1%% of each beamlet's max" % n_beamlets,
+            "histories": int(total), "batched_s": t_batched, "plain_s": t_plain, "speedup": t_plain / t_batched,
+            "batched_histories_per_s": total / t_batched, "plain_histories_per_s": total / t_plain,
+            "nnz": int(row_ptr[-1]),
+            "gamma_batched_fp32_vs_plain_fp64": {"beamlets": [int(i) for i in order], "pass_rates": rates,
+                                                 "criterion": "1%/1mm global, voxels > 10% of the beamlet max"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +117,7 @@ def main():
     ap.add_argument("--no-gamma", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--beamlets", type=int, default=512, help="beamlets of the batched leg (0: skip it)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -164,6 +206,8 @@ def main():
                                         "max_rel_dose_diff_above_cutoff":
                                             float(np.max(np.abs(dose[m] - d64[m]) / d64.max())),
                                         "fp64_histories_per_s": nhist / st64["seconds"]}
+    if args.beamlets > 0:
+        result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
     print(json.dumps(result), flush=True)
