@@ -65,7 +65,7 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
                       "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
 
 
-def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4):
+def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, tune=True):
     """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
     per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
     same history ids; only GPU calls are timed (no host thresholding for the
@@ -74,10 +74,13 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4):
     import phantoms
     bl = phantoms.beamlets(n_beamlets)
     total = sum(b[4] for b in bl)
+    sched = dpm_amd.autotune_beamlets(t, ph, bl, nsample=4000000) if tune else None
+    if sched is not None:
+        sched["chunk"] = 0
     # warm-up: same beamlets with few histories (allocates the per-beamlet rows)
     dpm_amd.simulate_beamlets(t, ph, [b[:4] + (100,) for b in bl], rel_threshold=1e-3, seed=seed)
     t0 = time.perf_counter()
-    row_ptr, col, val, st = dpm_amd.simulate_beamlets(t, ph, bl, rel_threshold=1e-3, seed=seed)
+    row_ptr, col, val, st = dpm_amd.simulate_beamlets(t, ph, bl, rel_threshold=1e-3, seed=seed, schedule=sched)
     t_batched = time.perf_counter() - t0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     offs = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
@@ -101,7 +104,7 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4):
                         "> 0.1%% of each beamlet's max" % n_beamlets,
             "histories": int(total), "batched_s": t_batched, "plain_s": t_plain, "speedup": t_plain / t_batched,
             "batched_histories_per_s": total / t_batched, "plain_histories_per_s": total / t_plain,
-            "nnz": int(row_ptr[-1]),
+            "nnz": int(row_ptr[-1]), "batched_schedule": sched,
             "gamma_batched_fp32_vs_plain_fp64": {"beamlets": [int(i) for i in order], "pass_rates": rates,
                                                  "criterion": "1%/1mm global, voxels > 10% of the beamlet max"}}
 
@@ -118,6 +121,7 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--beamlets", type=int, default=512, help="beamlets of the batched leg (0: skip it)")
+    ap.add_argument("--no-tune", action="store_true", help="default launch schedule instead of the autotuner's")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -155,10 +159,19 @@ def main():
     outs = {prec: (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
             for prec in (dpm_amd.DPM_FP32, dpm_amd.DPM_FP64)}
 
+    # launch schedule: the autotuner's choice for this source (outside the
+    # timed region), unless fixed on the command line
+    if args.no_tune or args.waves_per_cu or args.chunk:
+        sched = {"chunk": args.chunk, "waves_per_cu": args.waves_per_cu, "refill_min": 0, "tally_mode": 0,
+                 "source": "command line / defaults"}
+    else:
+        sched = dpm_amd.autotune(t, ph, nsample=2000000, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field)
+        sched["source"] = "dpm_autotune on 2e6 histories of this workload"
+        sched["chunk"] = 0
+
     def step(precision=dpm_amd.DPM_FP32, seed=1):
         return dpm_amd.simulate(t, ph, nhist, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=seed,
-                                precision=precision, nbatch=args.nbatch, chunk=args.chunk,
-                                waves_per_cu=args.waves_per_cu, out=outs[precision])
+                                precision=precision, nbatch=args.nbatch, out=outs[precision], schedule=sched)
 
     for _ in range(args.warmup):
         step()
@@ -196,6 +209,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches},
         "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
         "device_seconds_per_step": st["seconds"],
+        "schedule": sched,
         "counters": st["counters"],
     }
     if not args.no_gamma:
@@ -207,7 +221,7 @@ def main():
                                             float(np.max(np.abs(dose[m] - d64[m]) / d64.max())),
                                         "fp64_histories_per_s": nhist / st64["seconds"]}
     if args.beamlets > 0:
-        result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets)
+        result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets, tune=not args.no_tune)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
     print(json.dumps(result), flush=True)

@@ -623,6 +623,8 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.hist_base = opt->hist_begin;
   P.stats = t->stats;
   P.overflow = t->ctl + 5;
+  P.refill_min = std::min(64, std::max(1, opt->refill_min));
+  P.tally_mode = opt->tally_mode == 1 ? 1 : 0;
 
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
@@ -1078,6 +1080,115 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   }
   if (st) *st = local;
   return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Times every candidate schedule on the sample source `hs` (device time of
+// run_transport, best of two runs) and writes the fastest to *out.
+template <typename R>
+int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int rows, Workspace<R>& ws,
+                  Tables<R>& T, dpm_schedule* out) {
+  const uint64_t n = hs.count(0) + hs.count(1);
+  const int64_t nb = p->geom == GEOM_SLAB ? p->nbins : p->nvox;
+  if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)))
+    return set_error(DPM_E_NOMEM, "cannot allocate the tally");
+  static const int kWaves[] = {8, 12, 16};
+  static const int kRefill[] = {1, 4, 16};
+  static const int kTally[] = {0, 1};
+  dpm_schedule best;
+  std::memset(&best, 0, sizeof(best));
+  double best_s = 1e300, default_s = 0.0;
+  int cand = 0;
+  for (int w : kWaves)
+    for (int r : kRefill)
+      for (int tm : kTally) {
+        if (tm == 1 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
+        dpm_run_options opt;
+        std::memset(&opt, 0, sizeof(opt));
+        opt.seed = 0x5eed;
+        opt.hist_begin = 0;
+        opt.chunk = (int64_t)n;
+        opt.waves_per_cu = w;
+        opt.refill_min = r;
+        opt.tally_mode = tm;
+        double sec = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          (void)hipMemsetAsync(t->buf_rows.p, 0, sizeof(double) * nb * rows, t->stream);
+          if (p->geom == GEOM_SLAB) (void)hipMemsetAsync(t->buf_steps.p, 0, sizeof(double) * nb, t->stream);
+          (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream);
+          dpm_stats st;
+          std::memset(&st, 0, sizeof(st));
+          const int rc = run_transport<R>(t, p, hs, &opt, ws, T, t->buf_rows.d(), rows,
+                                          p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr, nullptr, nullptr,
+                                          nullptr, &st);
+          if (rc != DPM_OK) return rc;
+          sec = std::min(sec, st.seconds);
+        }
+        ++cand;
+        if (w == 16 && r == 1 && tm == 0) default_s = sec;
+        if (sec < best_s) {
+          best_s = sec;
+          best.waves_per_cu = w;
+          best.refill_min = r;
+          best.tally_mode = tm;
+        }
+      }
+  best.chunk = kDefaultChunk;
+  best.candidates = cand;
+  best.histories_per_s = (double)n / best_s;
+  best.default_histories_per_s = default_s > 0 ? (double)n / default_s : 0.0;
+  *out = best;
+  return DPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nsample, int precision,
+                 dpm_schedule* best) {
+  if (!t || !p || !src || !best) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (nsample < 1000 || nsample > ((uint64_t)1 << 24)) return set_error(DPM_E_INVALID, "sample of 1e3 .. 2^24 histories");
+  if (precision != DPM_FP32 && precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (src->kind != DPM_SRC_PENCIL && src->kind != DPM_SRC_FIELD) return set_error(DPM_E_INVALID, "unknown source kind");
+  int rc = check_source(t, src->particle, src->energy, src->emin, src->emax);
+  if (rc != DPM_OK) return rc;
+  HIP_TRY(hipSetDevice(t->device));
+  const HostSource hs = source_from(src, nsample, 0);
+  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, 1, t->wsd, t->td, best)
+                               : autotune_impl<float>(t, p, hs, 1, t->wsf, t->tf, best);
+}
+
+int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb, uint64_t nsample,
+                          int precision, dpm_schedule* best) {
+  if (!t || !p || !b || nb <= 0 || !best) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (p->geom != GEOM_VOXEL) return set_error(DPM_E_INVALID, "beamlets need a voxel phantom");
+  if (nb > (1 << 20)) return set_error(DPM_E_INVALID, "too many beamlets");
+  if (nsample < 1000 || nsample > ((uint64_t)1 << 24)) return set_error(DPM_E_INVALID, "sample of 1e3 .. 2^24 histories");
+  if (precision != DPM_FP32 && precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  // every beamlet, its history count scaled so that the total is ~nsample
+  uint64_t total = 0;
+  for (int i = 0; i < nb; ++i) {
+    int rc = check_source(t, b[i].particle, b[i].energy, 0, 0);
+    if (rc != DPM_OK) return rc;
+    total += b[i].nhist;
+  }
+  if (total == 0) return set_error(DPM_E_INVALID, "beamlets without histories");
+  HostSource hs;
+  uint64_t first = 0;
+  for (int i = 0; i < nb; ++i) {
+    const uint64_t n = std::max<uint64_t>(1, (uint64_t)((double)b[i].nhist * (double)nsample / (double)total));
+    hs.add(HostEntry{first, n, b[i].x0, b[i].y0, b[i].energy, b[i].particle, i});
+    first += n;
+  }
+  HIP_TRY(hipSetDevice(t->device));
+  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, nb, t->wsd, t->td, best)
+                               : autotune_impl<float>(t, p, hs, nb, t->wsf, t->tf, best);
 }
 
 int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val) {

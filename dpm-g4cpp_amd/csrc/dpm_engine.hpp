@@ -50,6 +50,8 @@ struct KParams {
   StatsDev* stats;
   int* overflow;
   int kind;                 // 0 photon kernel, 1 electron kernel (counter slot)
+  int refill_min;           // idle lanes that trigger a wave refill (1..64)
+  int tally_mode;           // voxel tally: 0 per-lane write-combining, 1 one atomic per deposit
 };
 
 // Primaries of one particle kind, enumerated in list order: entry i owns

@@ -182,7 +182,7 @@ struct WComb {
 template <int GEOM, class BlockAcc>
 __device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb& w, R e, const int box[3],
                                          uint32_t tag, uint32_t hist) {
-  if (GEOM == GEOM_SLAB) {
+  if (GEOM == GEOM_SLAB || P.tally_mode == 1) {
     score<GEOM>(a, P, e, box, tag, hist);
     return;
   }
@@ -495,7 +495,8 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
   for (;;) {
-    if (!exhausted)
+    // compaction interval: refill once refill_min lanes are idle (or none is busy)
+    if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
         else gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);
@@ -737,7 +738,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
   for (;;) {
-    if (!exhausted)
+    if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
         else gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);

@@ -22,8 +22,8 @@ COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "second
 
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
-            "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_last_error",
-            "dpm_abi_version")
+            "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
+            "dpm_autotune_beamlets", "dpm_last_error", "dpm_abi_version")
 
 
 class DpmError(RuntimeError):
@@ -39,7 +39,7 @@ class Source(ctypes.Structure):
 class RunOptions(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("hist_begin", ctypes.c_uint64), ("precision", ctypes.c_int),
                 ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
-                ("hist_edep", ctypes.c_void_p)]
+                ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -57,6 +57,15 @@ class Stats(ctypes.Structure):
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
                     kernel_calls=list(self.kernel_calls))
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int), ("refill_min", ctypes.c_int),
+                ("tally_mode", ctypes.c_int), ("candidates", ctypes.c_int), ("histories_per_s", ctypes.c_double),
+                ("default_histories_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Beamlet(ctypes.Structure):
@@ -92,6 +101,9 @@ def lib():
         L.dpm_sample.argtypes = [vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
         L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
         L.dpm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ctypes.POINTER(Schedule)]
+        L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
+                                            ctypes.POINTER(Schedule)]
         L.dpm_host_free.argtypes = [vp]
         _lib = L
     return _lib
@@ -209,7 +221,7 @@ def pinned_empty(n, dtype=np.float64):
 
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
-             out=None):
+             out=None, refill_min=0, tally_mode=0, schedule=None):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -218,8 +230,11 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
                  x0=x0, y0=y0, field_x=field, field_y=field)
     he = np.zeros(nhist) if per_history else None
+    if schedule is not None:
+        chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
+                                                      schedule["refill_min"], schedule["tally_mode"])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
-                     waves_per_cu=waves_per_cu, hist_edep=_ptr(he))
+                     waves_per_cu=waves_per_cu, hist_edep=_ptr(he), refill_min=refill_min, tally_mode=tally_mode)
     if out is not None:
         tally, steps = out
         for a in (tally, steps):
@@ -237,12 +252,36 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     return tally, steps, st.as_dict(), he
 
 
+def autotune(tables, phantom, nsample=2000000, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0,
+             x0=0.0, y0=0.0, precision=DPM_FP32):
+    """Fastest launch schedule for this source on this phantom (dpm_autotune); a dict usable as schedule=."""
+    src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
+                 x0=x0, y0=y0, field_x=field, field_y=field)
+    sch = Schedule()
+    _check(lib().dpm_autotune(tables.h, phantom.h, ctypes.byref(src), int(nsample), precision, ctypes.byref(sch)))
+    return sch.as_dict()
+
+
+def autotune_beamlets(tables, phantom, beamlets, nsample=4000000, precision=DPM_FP32):
+    """Fastest launch schedule for a beamlet set (history counts scaled to ~nsample in total)."""
+    arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
+                                       for b in beamlets])
+    sch = Schedule()
+    _check(lib().dpm_autotune_beamlets(tables.h, phantom.h, arr, len(beamlets), int(nsample), precision,
+                                       ctypes.byref(sch)))
+    return sch.as_dict()
+
+
 def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
-                      waves_per_cu=0):
+                      waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0):
     """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
     arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
                                        for b in beamlets])
-    opt = RunOptions(seed=seed, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu)
+    if schedule is not None:
+        chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
+                                                      schedule["refill_min"], schedule["tally_mode"])
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
+                     refill_min=refill_min, tally_mode=tally_mode)
     row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
     st = Stats()
     _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,

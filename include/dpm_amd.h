@@ -150,6 +150,8 @@ typedef struct dpm_run_options {
   int64_t chunk;                   /* histories per queue fill; 0 = automatic */
   int waves_per_cu;                /* persistent waves per CU; 0 = automatic */
   double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
+  int refill_min;                  /* idle lanes of a wave that trigger its refill (1..64); 0 = 1 */
+  int tally_mode;                  /* voxel tally: 0 per-lane write-combining, 1 atomic per deposit */
 } dpm_run_options;
 
 enum {
@@ -199,6 +201,31 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
                           const dpm_run_options* opt, double rel_threshold,
                           int64_t* row_ptr, dpm_stats* stats);
 int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val);
+
+/* ---------------------------------------------------------------------------
+ * Autotuner (no counterpart in the reference, whose transport has no launch
+ * schedule): times candidate schedules of the persistent kernels -- waves per
+ * CU, refill threshold (compaction interval) and voxel-tally mode -- on a
+ * sample of `nsample` histories of the given source (or beamlets, scaled to
+ * about nsample histories in total) and returns the fastest. The chunk is
+ * not sampled (larger chunks always shorten the drain tails): `chunk` is set
+ * to the largest the engine allows; callers clamp it to their run size.
+ * Copy the fields into dpm_run_options for the real run.
+ * ------------------------------------------------------------------------- */
+typedef struct dpm_schedule {
+  int64_t chunk;
+  int waves_per_cu;
+  int refill_min;
+  int tally_mode;
+  int candidates;                  /* schedules timed */
+  double histories_per_s;          /* device rate of the best on the sample */
+  double default_histories_per_s;  /* device rate of the default schedule on the sample */
+} dpm_schedule;
+
+int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nsample, int precision,
+                 dpm_schedule* best);
+int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb, uint64_t nsample,
+                          int precision, dpm_schedule* best);
 
 /* ---------------------------------------------------------------------------
  * Interaction samplers with explicit uniforms (the reference's own model

@@ -162,3 +162,34 @@ def test_beamlets_equal_plain_runs(gpu_tables0):
         vals = val[row_ptr[b]:row_ptr[b + 1]]
         assert np.array_equal(np.sort(cols), keep)
         np.testing.assert_allclose(vals, d[cols], rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_schedules_do_not_change_results(gpu_tables0):
+    # per-particle streams: chunk, waves per CU, refill threshold and tally
+    # mode change only the order of FP64 tally additions
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    kw = dict(particle=0, emin=0.5, emax=20.0, field=40.0, seed=17, per_history=True)
+    d0, _, _, h0 = dpm_amd.simulate(gpu_tables0, ph, 4000, **kw)
+    for sched in (dict(chunk=1000, waves_per_cu=8, refill_min=16, tally_mode=1),
+                  dict(chunk=0, waves_per_cu=12, refill_min=4, tally_mode=0),
+                  dict(chunk=0, waves_per_cu=16, refill_min=64, tally_mode=1)):
+        d, _, _, h = dpm_amd.simulate(gpu_tables0, ph, 4000, schedule=sched, **kw)
+        np.testing.assert_allclose(h, h0, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(d, d0, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_autotune_returns_a_timed_candidate(gpu_tables0):
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    s = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0)
+    assert s["waves_per_cu"] in (8, 12, 16) and s["refill_min"] in (1, 4, 16) and s["tally_mode"] in (0, 1)
+    assert s["candidates"] == 18 and s["chunk"] > 0
+    assert s["histories_per_s"] >= s["default_histories_per_s"] > 0
+    bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000)]
+    sb = dpm_amd.autotune_beamlets(gpu_tables0, ph, bl, nsample=5000)
+    assert sb["candidates"] == 18 and sb["histories_per_s"] > 0
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.autotune(gpu_tables0, ph, nsample=10, particle=0, emin=0.5, emax=20.0)
