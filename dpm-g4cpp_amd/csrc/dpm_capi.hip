@@ -305,6 +305,21 @@ struct Workspace {
   }
 };
 
+// most transport rounds enqueued per host synchronisation (and per captured
+// graph); rounds past the drain find empty queues and exit at once
+constexpr int kRoundsPerSync = 8;
+
+// round preparation as a kernel (not a memset: it is recorded into graphs):
+// clear the next round's queue counts and both consumption heads
+__global__ void k_round_prep(int* ctl, int nxt) {
+  if (threadIdx.x == 0) {
+    ctl[nxt] = 0;
+    ctl[2 + nxt] = 0;
+    ctl[4] = 0;
+    ctl[6] = 0;
+  }
+}
+
 // histories per queue fill when the caller does not choose
 constexpr int64_t kDefaultChunk = (int64_t)1 << 24;
 
@@ -430,7 +445,7 @@ struct dpm_tables {
   Workspace<double> wsd;
   int* ctl = nullptr;        // [0..3] queue counts, [4] head, [5] overflow
   StatsDev* stats = nullptr;
-  hipEvent_t ev[10] = {};
+  hipEvent_t ev[10 + 4 * kRoundsPerSync] = {};  // [10 + 4k ..]: kernel bounds of round k of a sync block
   // tally buffers reused across runs
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
   DevBuf buf_src;            // source description of the current run
@@ -654,6 +669,21 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   int rounds = 0;
   float ms_ph = 0, ms_el = 0, ms_fold = 0;
   int c_ph = 0, c_el = 0, c_fold = 0;
+  const bool use_graph = opt->graph != 0;
+  if (opt->rounds_per_sync != 0 && (opt->rounds_per_sync < 2 || opt->rounds_per_sync > kRoundsPerSync ||
+                                    (opt->rounds_per_sync & 1)))
+    return set_error(DPM_E_INVALID, "rounds_per_sync must be even, 2.." + std::to_string(kRoundsPerSync));
+  const int rps = opt->rounds_per_sync ? opt->rounds_per_sync : kRoundsPerSync;
+  hipGraphExec_t g_first = nullptr, g_rest = nullptr;
+  int n_graphs = 0;
+  struct GraphGuard {
+    hipGraphExec_t* a;
+    hipGraphExec_t* b;
+    ~GraphGuard() {
+      if (*a) (void)hipGraphExecDestroy(*a);
+      if (*b) (void)hipGraphExecDestroy(*b);
+    }
+  } guard{&g_first, &g_rest};
   while (done < nhist) {
     uint64_t n = std::min<uint64_t>((uint64_t)chunk, nhist - done);
     uint32_t bfirst = 0;
@@ -676,38 +706,69 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     const PrimRange r_el{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst};
     const PrimRange none{0, 0, bfirst};
     HIP_TRY(hipMemsetAsync(t->ctl, 0, 8 * sizeof(int), s));
-    int cur = 0;
+    // rps rounds of photons then electrons; the first generates
+    // this chunk's primaries. Round r reads queues `cur` and appends photons
+    // to `nxt`; electrons of both kernels go to the electron queue the
+    // electron kernel of the round (photons) or the next round (electrons) reads.
+    auto enqueue = [&](bool with_primaries, bool timed) -> hipError_t {
+      int c = 0;
+      for (int k = 0; k < rps; ++k) {
+        const int nx = c ^ 1;
+        hipLaunchKernelGGL(k_round_prep, dim3(1), dim3(64), 0, s, t->ctl, nx);
+        if (timed) (void)hipEventRecord(t->ev[10 + 4 * k], s);
+        P.kind = 0;
+        hipError_t e = Launch<R>::photons(P, dS, with_primaries && k == 0 ? r_ph : none, p->geom, blocks,
+                                          ws.q[0 + c], t->ctl + 4, ws.q[2 + c], ws.q[0 + nx], s);
+        if (e != hipSuccess) return e;
+        if (timed) { (void)hipEventRecord(t->ev[11 + 4 * k], s); (void)hipEventRecord(t->ev[12 + 4 * k], s); }
+        P.kind = 1;
+        e = Launch<R>::electrons(P, dS, with_primaries && k == 0 ? r_el : none, p->geom, blocks, ws.q[2 + c],
+                                 t->ctl + 6, ws.q[2 + nx], ws.q[0 + nx], s);
+        if (e != hipSuccess) return e;
+        if (timed) (void)hipEventRecord(t->ev[13 + 4 * k], s);
+        c = nx;
+      }
+      return hipSuccess;
+    };
     bool first = true;
     for (;;) {
-      const int nxt = cur ^ 1;
-      // clear next-round counts and the consumption head
-      HIP_TRY(hipMemsetAsync(t->ctl + nxt, 0, sizeof(int), s));
-      HIP_TRY(hipMemsetAsync(t->ctl + 2 + nxt, 0, sizeof(int), s));
-      HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
-      HIP_TRY(hipEventRecord(t->ev[3], s));
-      P.kind = 0;
-      HIP_TRY(Launch<R>::photons(P, dS, first ? r_ph : none, p->geom, blocks, ws.q[0 + cur], t->ctl + 4,
-                                 ws.q[2 + cur], ws.q[0 + nxt], s));
-      HIP_TRY(hipEventRecord(t->ev[4], s));
-      HIP_TRY(hipMemsetAsync(t->ctl + 4, 0, sizeof(int), s));
-      HIP_TRY(hipEventRecord(t->ev[5], s));
-      P.kind = 1;
-      HIP_TRY(Launch<R>::electrons(P, dS, first ? r_el : none, p->geom, blocks, ws.q[2 + cur], t->ctl + 4,
-                                   ws.q[2 + nxt], ws.q[0 + nxt], s));
-      HIP_TRY(hipEventRecord(t->ev[6], s));
+      if (use_graph) {
+        hipGraphExec_t& ge = first ? g_first : g_rest;
+        if (first && g_first) { (void)hipGraphExecDestroy(g_first); g_first = nullptr; }
+        if (!ge) {
+          hipGraph_t g = nullptr;
+          HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const hipError_t e = enqueue(first, false);
+          const hipError_t e2 = hipStreamEndCapture(s, &g);
+          if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_error(DPM_E_DEVICE, "graph capture of the transport rounds failed");
+          }
+          const hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          if (e3 != hipSuccess) { ge = nullptr; return set_error(DPM_E_DEVICE, "graph instantiation failed"); }
+          ++n_graphs;
+        }
+        HIP_TRY(hipGraphLaunch(ge, s));
+      } else {
+        HIP_TRY(enqueue(first, true));
+      }
       int counts[6];
       HIP_TRY(hipMemcpyAsync(counts, t->ctl, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
       first = false;
-      ms_ph += elapsed(t->ev[3], t->ev[4]);
-      ms_el += elapsed(t->ev[5], t->ev[6]);
-      ++c_ph; ++c_el;
-      ++rounds;
+      if (!use_graph)
+        for (int k = 0; k < rps; ++k) {
+          ms_ph += elapsed(t->ev[10 + 4 * k], t->ev[11 + 4 * k]);
+          ms_el += elapsed(t->ev[12 + 4 * k], t->ev[13 + 4 * k]);
+          ++c_ph; ++c_el;
+        }
+      rounds += rps;
       if (counts[5] != 0)
         return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
                                              "); use a smaller chunk");
-      if (counts[0 + nxt] == 0 && counts[2 + nxt] == 0) break;
-      cur = nxt;
+      // an even number of rounds leaves the next inputs in queues 0 and 2
+      if (counts[0] == 0 && counts[2] == 0) break;
       if (rounds > 100000) return set_error(DPM_E_DEVICE, "transport did not drain");
     }
     if (nbatch > 0) {
@@ -730,6 +791,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     // slot 0 (source) stays 0: primaries are generated inside the transport kernels
     st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
     st->kernel_calls[0] = 0; st->kernel_calls[1] = c_ph; st->kernel_calls[2] = c_el; st->kernel_calls[3] = c_fold;
+    st->graphs = n_graphs;
   }
   return DPM_OK;
 }

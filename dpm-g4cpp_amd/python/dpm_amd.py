@@ -39,7 +39,8 @@ class Source(ctypes.Structure):
 class RunOptions(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("hist_begin", ctypes.c_uint64), ("precision", ctypes.c_int),
                 ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
-                ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int)]
+                ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int),
+                ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -47,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 10),
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
-                ("counters_electron", ctypes.c_uint64 * 10)]
+                ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
@@ -56,7 +57,7 @@ class Stats(ctypes.Structure):
                     counters_photon=dict(zip(COUNTER_NAMES, list(self.counters_photon))),
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
-                    kernel_calls=list(self.kernel_calls))
+                    kernel_calls=list(self.kernel_calls), graphs=self.graphs)
 
 
 class Schedule(ctypes.Structure):
@@ -221,7 +222,7 @@ def pinned_empty(n, dtype=np.float64):
 
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
-             out=None, refill_min=0, tally_mode=0, schedule=None):
+             out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -234,7 +235,8 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
                                                       schedule["refill_min"], schedule["tally_mode"])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
-                     waves_per_cu=waves_per_cu, hist_edep=_ptr(he), refill_min=refill_min, tally_mode=tally_mode)
+                     waves_per_cu=waves_per_cu, hist_edep=_ptr(he), refill_min=refill_min, tally_mode=tally_mode,
+                     graph=int(bool(graph)), rounds_per_sync=rounds_per_sync)
     if out is not None:
         tally, steps = out
         for a in (tally, steps):
@@ -273,7 +275,7 @@ def autotune_beamlets(tables, phantom, beamlets, nsample=4000000, precision=DPM_
 
 
 def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
-                      waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0):
+                      waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0, graph=True):
     """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
     arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
                                        for b in beamlets])
@@ -281,7 +283,7 @@ def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, pre
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
                                                       schedule["refill_min"], schedule["tally_mode"])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
-                     refill_min=refill_min, tally_mode=tally_mode)
+                     refill_min=refill_min, tally_mode=tally_mode, graph=int(bool(graph)))
     row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
     st = Stats()
     _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,

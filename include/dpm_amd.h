@@ -152,6 +152,10 @@ typedef struct dpm_run_options {
   double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
   int refill_min;                  /* idle lanes of a wave that trigger its refill (1..64); 0 = 1 */
   int tally_mode;                  /* voxel tally: 0 per-lane write-combining, 1 atomic per deposit */
+  int graph;                       /* 1: replay the transport rounds from captured hipGraphs (no
+                                      per-kernel timings in dpm_stats) */
+  int rounds_per_sync;             /* transport rounds enqueued (or captured) per host synchronisation:
+                                      even, 2..8; 0 = 8 */
 } dpm_run_options;
 
 enum {
@@ -165,12 +169,14 @@ typedef struct dpm_stats {
   double e_primary;                /* MeV, sum of the primaries' kinetic energies */
   uint64_t n_pair, n_annih;
   uint64_t counters[DPM_NCNT];     /* byte-model counters */
-  int rounds;                      /* queue rounds executed */
+  int rounds;                      /* queue rounds enqueued (including empty ones after the drain) */
   int hist_size;                   /* slab: rows written by the reference rule */
-  double kernel_ms[4];             /* device time in source, photon, electron, tally-fold kernels */
+  double kernel_ms[4];             /* device time in source (0: primaries are generated by the transport
+                                      kernels), photon, electron, tally-fold kernels; eager runs only */
   int kernel_calls[4];             /* launches of each of those kernels */
   uint64_t counters_photon[DPM_NCNT];    /* counters of the photon kernel */
   uint64_t counters_electron[DPM_NCNT];  /* counters of the electron kernel */
+  int graphs;                      /* hipGraphs captured by the run (graph = 1) */
 } dpm_stats;
 
 /* Replaces Simulate(nprimary, eprimary, iselectron, lbox, mat, el, ph,

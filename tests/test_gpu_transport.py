@@ -193,3 +193,30 @@ def test_autotune_returns_a_timed_candidate(gpu_tables0):
     assert sb["candidates"] == 18 and sb["histories_per_s"] > 0
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.autotune(gpu_tables0, ph, nsample=10, particle=0, emin=0.5, emax=20.0)
+
+
+@pytest.mark.gpu
+def test_graph_replays_match_eager(gpu_tables0):
+    # 2 rounds per captured block and 3 chunks: the drain graph is replayed
+    # several times per call; eager runs in between, every replay checked
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    kw = dict(particle=0, emin=0.5, emax=20.0, field=40.0, seed=23, per_history=True, chunk=2000)
+    d0, _, st0, h0 = dpm_amd.simulate(gpu_tables0, ph, 6000, **kw)
+    assert st0["graphs"] == 0
+    for rep in range(3):
+        d, _, st, h = dpm_amd.simulate(gpu_tables0, ph, 6000, graph=True, rounds_per_sync=2, **kw)
+        assert st["graphs"] >= 4 and st["rounds"] >= 6, st
+        np.testing.assert_allclose(h, h0, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(d, d0, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(st["e_primary"], st0["e_primary"], rtol=1e-12)
+        dpm_amd.simulate(gpu_tables0, ph, 500, particle=-1, energy=12.0, seed=rep)   # eager work in between
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.simulate(gpu_tables0, ph, 100, particle=0, energy=5.0, rounds_per_sync=3)
+    bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000), (0.0, 0.0, 1, 15.0, 2000)]
+    ra, ca, va, _ = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, seed=8, graph=False)
+    for rep in range(2):
+        rb, cb, vb, sb = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, seed=8, graph=True)
+        assert sb["graphs"] >= 1
+        assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+        np.testing.assert_allclose(va, vb, rtol=1e-9)
