@@ -379,34 +379,85 @@ __global__ void k_sum(const double* x, int64_t n, double* out) {
   }
 }
 
-// per-row maximum of the beamlet tally (one block per row)
-__global__ void k_rowmax(const double* rows, int64_t nvox, double* rmax) {
-  __shared__ double red[256];
-  const double* r = rows + (size_t)blockIdx.x * nvox;
-  double m = 0;
-  for (int64_t v = threadIdx.x; v < nvox; v += blockDim.x) m = fmax(m, r[v]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rmax[blockIdx.x] = red[0];
+// Beamlet rows -> CSR in three passes over (row, segment) blocks, 1-D grid
+// block = row * kCsrSeg + segment:
+//   k_bl_stats  row maxima (of the non-negative part) and the sum of the rows
+//   k_bl_count  entries above threshold per segment
+//   k_bl_fill   entries written in voxel order at their segment's offset
+constexpr int kCsrSeg = 64;
+
+__device__ __forceinline__ void bl_segment(int64_t nvox, int64_t* v0, int64_t* v1) {
+  const int64_t seg = blockIdx.x % kCsrSeg;
+  const int64_t len = (nvox + kCsrSeg - 1) / kCsrSeg;
+  *v0 = seg * len;
+  *v1 = *v0 + len < nvox ? *v0 + len : nvox;
 }
 
-// count (write == 0) or write (write == 1) the entries above threshold of each
-// row; one block per row, entries written in voxel order
-__global__ void k_csr(const double* rows, int64_t nvox, const double* rmax, double rel, const int64_t* rowptr,
-                      int64_t* counts, int32_t* col, double* val, int write) {
+__global__ void k_bl_stats(const double* rows, int64_t nvox, unsigned long long* rmax_bits, double* sum) {
+  __shared__ double pa[4], pm[4];
+  const int row = blockIdx.x / kCsrSeg;
+  int64_t v0, v1;
+  bl_segment(nvox, &v0, &v1);
+  const double* r = rows + (size_t)row * nvox;
+  double acc = 0.0, m = 0.0;
+  for (int64_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+    const double x = r[v];
+    acc += x;
+    m = fmax(m, x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_xor(acc, o);
+    m = fmax(m, __shfl_xor(m, o));
+  }
+  if ((threadIdx.x & 63) == 0) { pa[threadIdx.x >> 6] = acc; pm[threadIdx.x >> 6] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, mm = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a += pa[w]; mm = fmax(mm, pm[w]); }
+    if (a != 0.0) unsafeAtomicAdd(sum, a);
+    // non-negative doubles order like their bit patterns
+    if (mm > 0.0) atomicMax(&rmax_bits[row], (unsigned long long)__double_as_longlong(mm));
+  }
+}
+
+__global__ void k_bl_count(const double* rows, int64_t nvox, const unsigned long long* rmax_bits, double rel,
+                           int64_t* segcnt) {
+  __shared__ int pc[4];
+  const int row = blockIdx.x / kCsrSeg;
+  int64_t v0, v1;
+  bl_segment(nvox, &v0, &v1);
+  const double* r = rows + (size_t)row * nvox;
+  const double thr = rel * __longlong_as_double((long long)rmax_bits[row]);
+  int c = 0;
+  for (int64_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+    const double x = r[v];
+    c += (x > thr && x > 0.0) ? 1 : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) pc[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t tsum = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tsum += pc[w];
+    segcnt[blockIdx.x] = tsum;
+  }
+}
+
+__global__ void k_bl_fill(const double* rows, int64_t nvox, const unsigned long long* rmax_bits, double rel,
+                          const int64_t* segoff, int32_t* col, double* val) {
   __shared__ int64_t base;
   __shared__ int wsum[4];
-  const double* r = rows + (size_t)blockIdx.x * nvox;
-  const double thr = rel * rmax[blockIdx.x];
-  if (threadIdx.x == 0) base = write ? rowptr[blockIdx.x] : 0;
+  const int row = blockIdx.x / kCsrSeg;
+  int64_t v0, v1;
+  bl_segment(nvox, &v0, &v1);
+  const double* r = rows + (size_t)row * nvox;
+  const double thr = rel * __longlong_as_double((long long)rmax_bits[row]);
+  if (threadIdx.x == 0) base = segoff[blockIdx.x];
   __syncthreads();
-  for (int64_t v0 = 0; v0 < nvox; v0 += blockDim.x) {
-    const int64_t v = v0 + threadIdx.x;
-    const bool keep = v < nvox && r[v] > thr && r[v] > 0.0;
+  for (int64_t b0 = v0; b0 < v1; b0 += blockDim.x) {
+    const int64_t v = b0 + threadIdx.x;
+    const double x = v < v1 ? r[v] : 0.0;
+    const bool keep = v < v1 && x > thr && x > 0.0;
     const uint64_t b = __ballot(keep);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) wsum[wave] = __popcll(b);
@@ -414,15 +465,14 @@ __global__ void k_csr(const double* rows, int64_t nvox, const double* rmax, doub
     int before = 0;
     for (int w = 0; w < wave; ++w) before += wsum[w];
     const int rank = before + (int)__popcll(b & ((1ull << (threadIdx.x & 63)) - 1ull));
-    if (keep && write) {
+    if (keep) {
       col[base + rank] = (int32_t)v;
-      val[base + rank] = r[v];
+      val[base + rank] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
   }
-  if (!write && threadIdx.x == 0) counts[blockIdx.x] = base;
 }
 
 }  // namespace
@@ -449,7 +499,7 @@ struct dpm_tables {
   // tally buffers reused across runs
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
   DevBuf buf_src;            // source description of the current run
-  DevBuf buf_blaux;          // beamlet row maxima, counts and row pointers
+  DevBuf buf_blaux;          // beamlet row maxima, segment counts and offsets
   // CSR of the last beamlet run
   int32_t* csr_col = nullptr;
   double* csr_val = nullptr;
@@ -573,6 +623,15 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   S.z0 = z0;
   S.hist_begin = hist_begin;
   S.nhist_total = nhist;
+  for (int k = 0; k < 2; ++k) {
+    S.count[k] = hs.count(k);
+    // several entries (beamlets): interleave their histories with a
+    // multiplicative bijection of the index (the prime must not divide the count)
+    S.perm[k] = 0;
+    if (hs.list[k].size() > 1 && S.count[k] > 1)
+      for (uint64_t prime : {2654435761ull, 4294967291ull, 2246822519ull})
+        if (S.count[k] % prime != 0) { S.perm[k] = prime; break; }
+  }
   for (int k = 0; k < 2; ++k) {
     const std::vector<HostEntry>& L = hs.list[k];
     const int n = (int)L.size();
@@ -1090,14 +1149,16 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   t->csr_nnz = 0;
   // dense per-beamlet rows (nb x nvox FP64), cached like the other tallies
   const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
-  if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * (nb + 1) + 1024))
+  const size_t nseg = (size_t)nb * kCsrSeg;
+  if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * nseg + 1024))
     return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string(rows_bytes >> 20) + " MiB)");
   double* d_rows = t->buf_rows.d();
   unsigned char* q = (unsigned char*)t->buf_blaux.p;
-  double* d_rmax = (double*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
-  int64_t* d_cnt = (int64_t*)q; q += ((sizeof(int64_t) * (nb + 1) + 255) & ~(size_t)255);
-  int64_t* d_rowptr = (int64_t*)q;
+  unsigned long long* d_rmax = (unsigned long long*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
+  int64_t* d_segcnt = (int64_t*)q; q += ((sizeof(int64_t) * nseg + 255) & ~(size_t)255);
+  int64_t* d_segoff = (int64_t*)q;
   (void)hipMemsetAsync(d_rows, 0, rows_bytes, s);
+  (void)hipMemsetAsync(d_rmax, 0, sizeof(double) * nb, s);
   (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), s);
   dpm_stats local;
   std::memset(&local, 0, sizeof(local));
@@ -1106,34 +1167,42 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
                ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
                : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
   if (rc == DPM_OK) {
-    hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, d_rows, (int64_t)nb * p->nvox,
-                       &t->stats->scored);
-    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
+    hipLaunchKernelGGL(k_bl_stats, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax, &t->stats->scored);
+    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "beamlet row statistics launch failed");
   }
   if (rc == DPM_OK) rc = finish_stats(t, &local);
   if (rc == DPM_OK) {
-    hipLaunchKernelGGL(k_rowmax, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax);
-    hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr, d_cnt,
-                       (int32_t*)nullptr, (double*)nullptr, 0);
-    std::vector<int64_t> cnt(nb);
+    hipLaunchKernelGGL(k_bl_count, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold,
+                       d_segcnt);
+    std::vector<int64_t> seg(nseg);
     // the run stream is non-blocking: copies are ordered on it
-    if (hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(seg.data(), d_segcnt, sizeof(int64_t) * nseg, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       rc = set_error(DPM_E_DEVICE, "CSR count failed");
     if (rc == DPM_OK) {
+      // segment counts -> global segment offsets and row pointers
+      int64_t acc = 0;
       row_ptr[0] = 0;
-      for (int i = 0; i < nb; ++i) row_ptr[i + 1] = row_ptr[i] + cnt[i];
-      const int64_t nnz = row_ptr[nb];
+      for (int i = 0; i < nb; ++i) {
+        for (int k = 0; k < kCsrSeg; ++k) {
+          const int64_t c = seg[(size_t)i * kCsrSeg + k];
+          seg[(size_t)i * kCsrSeg + k] = acc;
+          acc += c;
+        }
+        row_ptr[i + 1] = acc;
+      }
+      const int64_t nnz = acc;
       if (nnz > 0) {
         if (hipMalloc(&t->csr_col, sizeof(int32_t) * nnz) != hipSuccess ||
             hipMalloc(&t->csr_val, sizeof(double) * nnz) != hipSuccess) {
           rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
-        } else if (hipMemcpyAsync(d_rowptr, row_ptr, sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s) !=
+        } else if (hipMemcpyAsync(d_segoff, seg.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s) !=
                    hipSuccess) {
-          rc = set_error(DPM_E_DEVICE, "CSR row pointer copy failed");
+          rc = set_error(DPM_E_DEVICE, "CSR offset copy failed");
         } else {
-          hipLaunchKernelGGL(k_csr, dim3(nb), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold, d_rowptr,
-                             d_cnt, t->csr_col, t->csr_val, 1);
+          hipLaunchKernelGGL(k_bl_fill, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax,
+                             rel_threshold, d_segoff, t->csr_col, t->csr_val);
+          // the staging vector must outlive the copy above
           if (hipStreamSynchronize(s) != hipSuccess) rc = set_error(DPM_E_DEVICE, "CSR build failed");
           else t->csr_nnz = nnz;
         }

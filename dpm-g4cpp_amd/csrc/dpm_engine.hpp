@@ -77,6 +77,10 @@ struct SourceParams {
   double fx, fy, z0;
   uint64_t hist_begin, nhist_total;
   PrimList list[2];         // [0] photons, [1] electrons and positrons
+  uint64_t perm[2];         // > 0: index v of list k is processed as (v * perm[k]) mod (its count), a
+                            // bijection (perm prime, not a divisor of the count) that interleaves the
+                            // entries' histories; 0: list order
+  uint64_t count[2];        // histories of each list
 };
 
 // primaries a transport launch generates after its input queue: this kind's

@@ -297,7 +297,10 @@ __device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Tra
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
                                             const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
-                                            double& eprim) {
+                                            double& eprim, int k) {
+  // beamlets: neighbouring lanes take histories of different entries, so that
+  // one pencil beam's axis voxels do not receive a wave's worth of atomics
+  if (S->perm[k]) v = (v * S->perm[k]) % S->count[k];
   int lo = 0, hi = L.n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -499,7 +502,7 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
-        else gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);
+        else gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 0);
         t.rng.topup();
         return start_track<GEOM>(acc, P, t);
       });
@@ -741,7 +744,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
-        else gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim);
+        else gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 1);
         t.rng.topup();
         return electron_init<GEOM>(acc, P, t, s);
       });
