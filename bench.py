@@ -5,14 +5,18 @@ Workload (one "step"): 1e7 photon histories in a homogeneous water cube of
 150^3 voxels of 2 mm (density 1.0), parallel 10x10 cm^2 field on the -z face,
 energies uniform in [0.5, 20] MeV, FP32 transport with an FP64 tally split in
 100 history batches (per-voxel sum and sum of squares). Inputs (tables,
-phantom) are resident on the GPU before the timed region; the timed region
-covers the whole transport of the step and the return of the dose.
+phantom) are resident on the GPU before the timed region; `value` times the
+whole job with its result (dose, sum of squares) left in HBM, and
+`pcie_inclusive` the same step returning both arrays to pinned host memory.
+The launch schedule is the autotuner's (chosen before the timed region).
 
 Printed: one JSON line with histories/s, the roofline of the dominant kernel
 (algorithmic bytes from the engine's event counters over that kernel's device
-time measured with HIP events on its stream), the 1 %/1 mm gamma pass rate of
-the FP32 dose against an FP64-transport GPU run of the same histories, and a
-CPU baseline (the FP64 oracle on this host's cores, a bounded sample).
+time measured with HIP events on its stream; HBM traffic from the committed
+rocprof summary), the 1 %/1 mm gamma pass rate of the FP32 dose against an
+FP64-transport GPU run of the same histories, the 512-beamlet batch timed
+plain vs batched, and a CPU baseline (the FP64 oracle on this host's cores, a
+bounded sample).
 """
 import argparse
 import json
@@ -65,7 +69,7 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
                       "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
 
 
-def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, tune=True):
+def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, sched=None, plain=True):
     """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
     per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
     same history ids; only GPU calls are timed (no host thresholding for the
@@ -74,9 +78,10 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
     import phantoms
     bl = phantoms.beamlets(n_beamlets)
     total = sum(b[4] for b in bl)
-    sched = dpm_amd.autotune_beamlets(t, ph, bl, nsample=4000000) if tune else None
-    if sched is not None:
+    if sched is None:
+        sched = dpm_amd.autotune_beamlets(t, ph, bl, nsample=4000000)
         sched["chunk"] = 0
+        sched["source"] = "dpm_autotune_beamlets on ~4e6 histories of this beamlet set"
     # warm-up: same beamlets with few histories (allocates the per-beamlet rows)
     dpm_amd.simulate_beamlets(t, ph, [b[:4] + (100,) for b in bl], rel_threshold=1e-3, seed=seed)
     t0 = time.perf_counter()
@@ -84,10 +89,13 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
     t_batched = time.perf_counter() - t0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     offs = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
-    t0 = time.perf_counter()
-    for i, (x, y, p, e, n) in enumerate(bl):
-        dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed, hist_begin=int(offs[i]), out=out)
-    t_plain = time.perf_counter() - t0
+    t_plain = None
+    if plain:
+        t0 = time.perf_counter()
+        for i, (x, y, p, e, n) in enumerate(bl):
+            dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed, hist_begin=int(offs[i]),
+                             out=out)
+        t_plain = time.perf_counter() - t0
     # correctness: batched FP32 rows of the largest beamlets vs FP64 plain runs
     order = np.argsort([-b[4] for b in bl])[:n_check]
     rates = []
@@ -102,8 +110,10 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
     return {"workload": "config4: %d beamlets on the 150^3 water phantom (e- 6-18 MeV / gamma U(1,20) MeV pencils "
                         "on a 2 mm grid over 160x160 mm2, lognormal history counts), FP32 transport, CSR of voxels "
                         "> 0.1%% of each beamlet's max" % n_beamlets,
-            "histories": int(total), "batched_s": t_batched, "plain_s": t_plain, "speedup": t_plain / t_batched,
-            "batched_histories_per_s": total / t_batched, "plain_histories_per_s": total / t_plain,
+            "histories": int(total), "batched_s": t_batched, "plain_s": t_plain,
+            "speedup": t_plain / t_batched if t_plain else None,
+            "batched_histories_per_s": total / t_batched,
+            "plain_histories_per_s": total / t_plain if t_plain else None,
             "nnz": int(row_ptr[-1]), "batched_schedule": sched,
             "gamma_batched_fp32_vs_plain_fp64": {"beamlets": [int(i) for i in order], "pass_rates": rates,
                                                  "criterion": "1%/1mm global, voxels > 10% of the beamlet max"}}
@@ -122,6 +132,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--beamlets", type=int, default=512, help="beamlets of the batched leg (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="default launch schedule instead of the autotuner's")
+    ap.add_argument("--refill-min", type=int, default=0, help="fix the refill threshold (skips the autotuner)")
+    ap.add_argument("--tally-mode", type=int, default=-1, help="fix the voxel-tally mode (skips the autotuner)")
+    ap.add_argument("--no-plain", action="store_true", help="beamlet leg: skip the plain per-beamlet runs")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -161,25 +174,34 @@ def main():
 
     # launch schedule: the autotuner's choice for this source (outside the
     # timed region), unless fixed on the command line
-    if args.no_tune or args.waves_per_cu or args.chunk:
-        sched = {"chunk": args.chunk, "waves_per_cu": args.waves_per_cu, "refill_min": 0, "tally_mode": 0,
-                 "source": "command line / defaults"}
+    fixed = args.no_tune or args.waves_per_cu or args.chunk or args.refill_min or args.tally_mode >= 0
+    if fixed:
+        sched = {"chunk": args.chunk, "waves_per_cu": args.waves_per_cu, "refill_min": args.refill_min,
+                 "tally_mode": max(0, args.tally_mode), "source": "command line / defaults"}
     else:
         sched = dpm_amd.autotune(t, ph, nsample=2000000, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field)
         sched["source"] = "dpm_autotune on 2e6 histories of this workload"
         sched["chunk"] = 0
 
-    def step(precision=dpm_amd.DPM_FP32, seed=1):
+    def step(precision=dpm_amd.DPM_FP32, seed=1, fetch=True):
         return dpm_amd.simulate(t, ph, nhist, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=seed,
-                                precision=precision, nbatch=args.nbatch, out=outs[precision], schedule=sched)
+                                precision=precision, nbatch=args.nbatch, out=outs[precision], schedule=sched,
+                                fetch=fetch)
 
     for _ in range(args.warmup):
-        step()
+        step(fetch=False)
+    # value: the whole job with the dose and its batch statistics left in HBM
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        dose, sum2, st, _ = step()
+        _, _, st, _ = step(fetch=False)
     wall = time.perf_counter() - t0
     value = args.steps * nhist / wall
+    dose, sum2 = dpm_amd.tally_fetch(t, ph, out=outs[dpm_amd.DPM_FP32])
+    # the same job including the copy of dose and sum of squares to (pinned) host memory
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    wall_pcie = time.perf_counter() - t0
 
     # roofline of the dominant kernel (device time from HIP events on the run stream)
     ms_ph, ms_el = st["kernel_ms"][1], st["kernel_ms"][2]
@@ -209,6 +231,9 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches},
         "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
         "device_seconds_per_step": st["seconds"],
+        "pcie_inclusive": {"value": args.steps * nhist / wall_pcie, "ms_per_step": wall_pcie / args.steps * 1e3,
+                           "what": "the same step returning dose and sum of squares (2 x %d FP64) to pinned "
+                                   "host memory" % ph.nbins},
         "schedule": sched,
         "counters": st["counters"],
     }
@@ -221,7 +246,8 @@ def main():
                                             float(np.max(np.abs(dose[m] - d64[m]) / d64.max())),
                                         "fp64_histories_per_s": nhist / st64["seconds"]}
     if args.beamlets > 0:
-        result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets, tune=not args.no_tune)
+        result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets,
+                                         sched=None if not fixed else sched, plain=not args.no_plain)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
     print(json.dumps(result), flush=True)

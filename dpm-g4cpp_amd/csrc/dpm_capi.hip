@@ -500,6 +500,10 @@ struct dpm_tables {
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
   DevBuf buf_src;            // source description of the current run
   DevBuf buf_blaux;          // beamlet row maxima, segment counts and offsets
+  // result of the last dpm_simulate, kept on the device for dpm_tally_fetch
+  const double* last_tally = nullptr;
+  const double* last_second = nullptr;
+  int64_t last_n = 0;
   // CSR of the last beamlet run
   int32_t* csr_col = nullptr;
   double* csr_val = nullptr;
@@ -1047,7 +1051,7 @@ int64_t dpm_phantom_num_bins(const dpm_phantom* p) { return p ? (p->geom == GEOM
 
 int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
                  const dpm_run_options* opt_in, double* tally, double* steps, dpm_stats* st) {
-  if (!t || !p || !src || !tally) return set_error(DPM_E_INVALID, "null argument");
+  if (!t || !p || !src) return set_error(DPM_E_INVALID, "null argument");
   if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
   if (nhist == 0) return set_error(DPM_E_INVALID, "zero histories");
   if (nhist > 0xFFFFFFFFull) return set_error(DPM_E_INVALID, "at most 2^32-1 histories per call");
@@ -1101,21 +1105,30 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
     if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
   }
   if (rc == DPM_OK) rc = finish_stats(t, &local);
-  if (rc == DPM_OK) {
-    const double* res = nbatch > 0 ? d_sum : d_rows;
-    if (hipMemcpy(tally, res, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally copy failed");
-    if (rc == DPM_OK && steps) {
-      const double* s2 = slab ? d_steps : d_sum2;
-      if (s2 && hipMemcpy(steps, s2, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess)
-        rc = set_error(DPM_E_DEVICE, "tally copy failed");
-      if (!s2) std::memset(steps, 0, sizeof(double) * nb);
-    }
-    if (rc == DPM_OK && opt.hist_edep &&
-        hipMemcpy(opt.hist_edep, d_he, sizeof(double) * nhist, hipMemcpyDeviceToHost) != hipSuccess)
-      rc = set_error(DPM_E_DEVICE, "per-history copy failed");
-  }
+  // the result stays on the device until the next run (dpm_tally_fetch)
+  t->last_tally = rc == DPM_OK ? (nbatch > 0 ? d_sum : d_rows) : nullptr;
+  t->last_second = rc == DPM_OK ? (slab ? d_steps : d_sum2) : nullptr;
+  t->last_n = rc == DPM_OK ? nb : 0;
+  if (rc == DPM_OK && tally) rc = dpm_tally_fetch(t, tally, steps);
+  if (rc == DPM_OK && opt.hist_edep &&
+      hipMemcpy(opt.hist_edep, d_he, sizeof(double) * nhist, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = set_error(DPM_E_DEVICE, "per-history copy failed");
   if (st) *st = local;
   return rc;
+}
+
+int dpm_tally_fetch(dpm_tables* t, double* tally, double* second) {
+  if (!t || !tally) return set_error(DPM_E_INVALID, "null argument");
+  if (!t->last_tally) return set_error(DPM_E_INVALID, "no tally on the device (run dpm_simulate first)");
+  HIP_TRY(hipSetDevice(t->device));
+  const size_t bytes = sizeof(double) * (size_t)t->last_n;
+  HIP_TRY(hipMemcpyAsync(tally, t->last_tally, bytes, hipMemcpyDeviceToHost, t->stream));
+  if (second) {
+    if (t->last_second) HIP_TRY(hipMemcpyAsync(second, t->last_second, bytes, hipMemcpyDeviceToHost, t->stream));
+    else std::memset(second, 0, bytes);
+  }
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  return DPM_OK;
 }
 
 int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb,
@@ -1148,6 +1161,8 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   t->csr_val = nullptr;
   t->csr_nnz = 0;
   // dense per-beamlet rows (nb x nvox FP64), cached like the other tallies
+  t->last_tally = t->last_second = nullptr;
+  t->last_n = 0;
   const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
   const size_t nseg = (size_t)nb * kCsrSeg;
   if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * nseg + 1024))

@@ -23,7 +23,7 @@ COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "second
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
-            "dpm_autotune_beamlets", "dpm_last_error", "dpm_abi_version")
+            "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_last_error", "dpm_abi_version")
 
 
 class DpmError(RuntimeError):
@@ -102,6 +102,7 @@ def lib():
         L.dpm_sample.argtypes = [vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
         L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
         L.dpm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.dpm_tally_fetch.argtypes = [vp, vp, vp]
         L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ctypes.POINTER(Schedule)]
         L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
                                             ctypes.POINTER(Schedule)]
@@ -222,12 +223,13 @@ def pinned_empty(n, dtype=np.float64):
 
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
-             out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0):
+             out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
     phantom.nbins elements to write into (e.g. from pinned_empty), returned
-    reshaped to the phantom's shape."""
+    reshaped to the phantom's shape. fetch=False leaves the result on the
+    device (tally and steps are returned as None; see tally_fetch)."""
     src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
                  x0=x0, y0=y0, field_x=field, field_y=field)
     he = np.zeros(nhist) if per_history else None
@@ -246,12 +248,25 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
         tally = np.zeros(phantom.nbins)
         steps = np.zeros(phantom.nbins)
     st = Stats()
+    if not fetch:
+        _check(lib().dpm_simulate(tables.h, phantom.h, ctypes.byref(src), nhist, ctypes.byref(opt), None, None,
+                                  ctypes.byref(st)))
+        return None, None, st.as_dict(), he
     _check(lib().dpm_simulate(tables.h, phantom.h, ctypes.byref(src), nhist, ctypes.byref(opt), _ptr(tally),
                               _ptr(steps), ctypes.byref(st)))
     if phantom.shape is not None:
         tally = tally.reshape(phantom.shape)
         steps = steps.reshape(phantom.shape)
     return tally, steps, st.as_dict(), he
+
+
+def tally_fetch(tables, phantom, out=None):
+    """The last run's (tally, steps_or_sum2) from the device (dpm_tally_fetch)."""
+    tally, second = out if out is not None else (np.zeros(phantom.nbins), np.zeros(phantom.nbins))
+    _check(lib().dpm_tally_fetch(tables.h, _ptr(tally), _ptr(second)))
+    if phantom.shape is not None:
+        return tally.reshape(phantom.shape), second.reshape(phantom.shape)
+    return tally, second
 
 
 def autotune(tables, phantom, nsample=2000000, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0,

@@ -186,9 +186,17 @@ typedef struct dpm_stats {
  *          steps[nbins] = number of scoring calls per bin (may be NULL);
  *   voxel: tally[nbins] = energy per voxel (MeV); with nbatch>1, steps (may be
  *          NULL) receives the per-voxel sum over batches of the squared batch
- *          tally. */
+ *          tally.
+ * tally may be NULL: the result then stays on the device (until the next run
+ * on this handle) and dpm_tally_fetch copies it out. */
 int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
                  const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
+
+/* Copies the tally of the last dpm_simulate on this handle (and, if
+ * `second` is not NULL, its steps / sum-of-squares array) into host arrays of
+ * dpm_phantom_num_bins elements. No counterpart in the reference (which writes
+ * hist.sim directly); it separates the run from the result transfer. */
+int dpm_tally_fetch(dpm_tables* t, double* tally, double* second);
 
 /* Batched beamlets: `nb` pencil beamlets (entry point, particle, energy,
  * history count) in one run, scored into a sparse dose-influence matrix:

@@ -220,3 +220,21 @@ def test_graph_replays_match_eager(gpu_tables0):
         assert sb["graphs"] >= 1
         assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
         np.testing.assert_allclose(va, vb, rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_result_left_on_device_then_fetched(gpu_tables0):
+    mat, rho = water_cube(20)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    kw = dict(particle=0, emin=0.5, emax=20.0, field=30.0, seed=31, nbatch=10)
+    d0, s0, st0, _ = dpm_amd.simulate(gpu_tables0, ph, 3000, **kw)
+    none, none2, st1, _ = dpm_amd.simulate(gpu_tables0, ph, 3000, fetch=False, **kw)
+    assert none is None and none2 is None
+    d1, s1 = dpm_amd.tally_fetch(gpu_tables0, ph)
+    np.testing.assert_allclose(d1, d0, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(s1, s0, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(st1["e_scored"], d1.sum(), rtol=1e-12)
+    # a beamlet run reuses the tally buffers: nothing left to fetch
+    dpm_amd.simulate_beamlets(gpu_tables0, ph, [(0.0, 0.0, -1, 9.0, 500)], seed=2)
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.tally_fetch(gpu_tables0, ph)
