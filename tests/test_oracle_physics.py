@@ -5,7 +5,10 @@
 * Philox-mode results do not depend on history order or chunking: a run of
   histories [0, N) equals the concatenation of runs [0, k) and [k, N);
 * a planted mistake in the oracle (dropping the Compton electron) must break
-  energy conservation -- the balance check is sensitive.
+  energy conservation -- the balance check is sensitive;
+* the voxel phantom (an extension of the reference's Geom) reduces to the
+  reference slab geometry when its voxels and materials replicate the slab's
+  layers: the same Philox histories give the same depth dose.
 """
 import numpy as np
 import pytest
@@ -60,3 +63,32 @@ def test_planted_mistake_breaks_conservation(tables3):
     finally:
         oracle_lib.set_fault(0)
     assert abs(balance(r, n * 5.0)) > 1e-3 * n * 5.0
+
+
+def test_voxel_geometry_reproduces_the_reference_slab(tables3):
+    # special case that reduces the voxel phantom (an extension) to the
+    # reference geometry: 1 mm voxels whose faces coincide with the slab's box
+    # faces, materials and nominal densities by depth index as Geom.cc:133-164.
+    # Same Philox histories; the two coordinate systems differ by a shift, so
+    # rounding can send a history another way: the depth doses agree closely
+    # but not bit for bit.
+    n, lbox, nxy, nz = 6000, 1.0, 201, 101
+    slab = oracle_lib.run_slab(tables3, n, e0=15.0, ptype=-1, lbox=lbox, geom_index=3,
+                               rng=oracle_lib.RNG_PHILOX, seed=29, per_history=True)
+    mats = np.array([max(oracle_lib.slab_layer_material(iz, lbox, 3), 0) for iz in range(nz)], dtype=np.uint8)
+    mat = np.broadcast_to(mats[:, None, None], (nz, nxy, nxy)).copy()
+    rho = np.array([1.0, 4.54, 1.85], dtype=np.float32)[mat]
+    vox = oracle_lib.run_voxel(tables3, mat, rho, lbox, n, ptype=-1, e0=15.0, seed=29, per_history=True)
+    same = np.abs(vox["hist_edep"] - slab["hist_edep"]) <= 1e-9 * np.abs(slab["hist_edep"])
+    assert np.mean(same) >= 0.9, np.mean(same)
+    depth_vox = vox["edep"].reshape(nz, nxy, nxy).sum(axis=(1, 2))
+    depth_slab = slab["edep"][:nz]
+    m = depth_slab > 0.2 * depth_slab.max()
+    np.testing.assert_allclose(depth_vox[m], depth_slab[m], rtol=0.01)
+    np.testing.assert_allclose(vox["e_scored"], slab["e_scored"], rtol=1e-3)
+    # and a planted mistake in the phantom (Ti layer given water's density) is seen
+    rho_bad = rho.copy()
+    rho_bad[mat == 1] = 1.0
+    bad = oracle_lib.run_voxel(tables3, mat, rho_bad, lbox, n, ptype=-1, e0=15.0, seed=29)
+    depth_bad = bad["edep"].reshape(nz, nxy, nxy).sum(axis=(1, 2))
+    assert np.max(np.abs(depth_bad[m] - depth_slab[m]) / depth_slab[m]) > 0.1
