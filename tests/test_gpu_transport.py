@@ -238,3 +238,48 @@ def test_result_left_on_device_then_fetched(gpu_tables0):
     dpm_amd.simulate_beamlets(gpu_tables0, ph, [(0.0, 0.0, -1, 9.0, 500)], seed=2)
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.tally_fetch(gpu_tables0, ph)
+
+
+def small_heterogeneous(shape=(24, 20, 16), seed=0):
+    rng = np.random.default_rng(seed)
+    mat = rng.integers(0, 3, size=shape).astype(np.uint8)
+    rho = (np.array([1.0, 4.54, 1.85], dtype=np.float32)[mat] *
+           rng.uniform(0.9, 1.1, mat.shape).astype(np.float32))
+    return mat, rho
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ptype,e0", [(-1, 12.0), (0, 8.0)])
+def test_heterogeneous_fp64_history_by_history(gpu_tables3, tables3, ptype, e0):
+    # water / Ti / bone voxels with density jitter (majorant scaled above 1)
+    mat, rho = small_heterogeneous()
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 2.0)
+    scale = float(np.max(rho / np.array([1.0, 4.54, 1.85])[mat]))
+    n = 1500
+    tally, _, st, he = dpm_amd.simulate(gpu_tables3, ph, n, particle=ptype, energy=e0, field=20.0, seed=41,
+                                        precision=dpm_amd.DPM_FP64, per_history=True)
+    ref = oracle_lib.run_voxel(tables3, mat, rho, 2.0, n, ptype=ptype, e0=e0, field=20.0, seed=41,
+                               per_history=True, majorant_scale=scale)
+    rel = np.abs(he - ref["hist_edep"]) / np.maximum(np.abs(ref["hist_edep"]), 1e-300)
+    ok = (rel < 1e-9) | ((np.abs(he) < 1e-300) & (np.abs(ref["hist_edep"]) < 1e-300))
+    assert np.mean(ok) >= 0.99, np.mean(ok)
+    bal = _energy_balance(n, e0, st["e_scored"], st["e_escaped"], st["n_pair"], st["n_annih"])
+    assert abs(bal) < 1e-9 * n * e0, bal
+
+
+@pytest.mark.gpu
+def test_config3_recipe_fp32_vs_fp64_gamma(gpu_tables3):
+    # the heterogeneous recipe (bone and Ti ellipsoids, lognormal density
+    # jitter) at reduced size, 12 MeV e- 10x10 cm2 field
+    import phantoms
+    mat, rho = phantoms.heterogeneous(64, 64, 48, 2.0, seed=1234, axis_scale=0.5)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 2.0)
+    n = 1000000
+    kw = dict(particle=-1, energy=12.0, field=100.0, seed=43, nbatch=10)
+    d32, s32, st32, _ = dpm_amd.simulate(gpu_tables3, ph, n, **kw)
+    d64, _, st64, _ = dpm_amd.simulate(gpu_tables3, ph, n, precision=dpm_amd.DPM_FP64, **kw)
+    dose32, dose64 = d32 / rho, d64 / rho
+    pr = gamma.gamma_pass_rate_3d(dose64, dose32, spacing_mm=2.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    assert pr >= 0.99, pr
+    bal = _energy_balance(n, 12.0, st32["e_scored"], st32["e_escaped"], st32["n_pair"], st32["n_annih"])
+    assert abs(bal) < 1e-5 * n * 12.0, bal
