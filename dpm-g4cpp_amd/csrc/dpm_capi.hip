@@ -1375,10 +1375,12 @@ int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* ener
   int* dab = (int*)take(4 * (size_t)n);
   int rc = DPM_OK;
   hipError_t e = hipMemcpy(de, energy, 8 * (size_t)n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(dm, 0, 4 * (size_t)n);
+  // memsets on the run stream: a null-stream memset is not ordered before a
+  // launch on the non-blocking run stream
+  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemsetAsync(dm, 0, 4 * (size_t)n, s);
   if (e == hipSuccess) e = hipMemcpy(d1, r1, 4 * (size_t)n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d2, r2, 4 * (size_t)n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = r3 ? hipMemcpy(d3, r3, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(d3, 0, 4 * (size_t)n);
+  if (e == hipSuccess) e = r3 ? hipMemcpy(d3, r3, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemsetAsync(d3, 0, 4 * (size_t)n, s);
   if (e == hipSuccess)
     e = precision == DPM_FP64 ? launch_sample_f64(t->td, kind, n, de, dm, d1, d2, d3, dout, deb, dab, s)
                               : launch_sample_f32(t->tf, kind, n, de, dm, d1, d2, d3, dout, deb, dab, s);
@@ -1389,6 +1391,33 @@ int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* ener
   if (e != hipSuccess) rc = set_error(DPM_E_DEVICE, std::string("dpm_sample: ") + hipGetErrorString(e));
   (void)hipFree(buf);
   return rc;
+}
+
+int dpm_perform(dpm_tables* t, int kind, int precision, int n, double energy, int imat, uint64_t seed,
+                double* out_value, double* out_cos) {
+  if (!t || !out_value || !out_cos || n < 0) return set_error(DPM_E_INVALID, "null argument");
+  if (kind < 0 || kind > 3) return set_error(DPM_E_INVALID, "unknown interaction");
+  if (precision != DPM_FP32 && precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (kind == 0 && (imat < 0 || imat >= t->nmat)) return set_error(DPM_E_INVALID, "material index out of range");
+  // the interaction must be possible at this energy (SimDPMLike.cc:174, 206)
+  const double lo = kind == 0 ? t->gcut : (kind == 1 ? 2.0 * t->ecut : (kind == 2 ? t->ecut : t->gcut));
+  if (!(energy > lo) || !(energy < 21.0)) return set_error(DPM_E_INVALID, "energy outside the interaction's range");
+  if (n == 0) return DPM_OK;
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  void* buf = nullptr;
+  const size_t half = ((size_t)n * 8 + 255) & ~(size_t)255;
+  HIP_TRY(hipMalloc(&buf, 2 * half));
+  double* dv = (double*)buf;
+  double* dc = (double*)((unsigned char*)buf + half);
+  hipError_t e = precision == DPM_FP64 ? launch_perform_f64(t->td, kind, n, energy, imat, seed, dv, dc, s)
+                                       : launch_perform_f32(t->tf, kind, n, energy, imat, seed, dv, dc, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_value, dv, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_cos, dc, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return set_error(DPM_E_DEVICE, std::string("dpm_perform: ") + hipGetErrorString(e));
+  return DPM_OK;
 }
 
 int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energy, const int* imat, double* out) {
@@ -1408,7 +1437,9 @@ int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energ
   double* dout = (double*)((unsigned char*)dm + (((size_t)n * 4 + 255) & ~(size_t)255));
   int rc = DPM_OK;
   hipError_t e = hipMemcpy(de, energy, 8 * (size_t)n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemset(dm, 0, 4 * (size_t)n);
+  // memsets on the run stream: a null-stream memset is not ordered before a
+  // launch on the non-blocking run stream
+  if (e == hipSuccess) e = imat ? hipMemcpy(dm, imat, 4 * (size_t)n, hipMemcpyHostToDevice) : hipMemsetAsync(dm, 0, 4 * (size_t)n, s);
   if (e == hipSuccess)
     e = precision == DPM_FP64 ? launch_eval_f64(t->td, which, n, de, dm, dout, s) : launch_eval_f32(t->tf, which, n, de, dm, dout, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -97,6 +97,8 @@ struct PrimRange {
   hipError_t launch_electrons##SFX(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int geom,     \
                                    int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,          \
                                    const QueueView<R>& qg, hipStream_t st);                                        \
+  hipError_t launch_perform##SFX(const Tables<R>& T, int kind, int n, double energy, int imat, uint64_t seed,     \
+                                 double* out_v, double* out_c, hipStream_t st);                                    \
   hipError_t launch_sample##SFX(const Tables<R>& T, int kind, int n, const double* e, const int* imat,             \
                                 const uint32_t* r1, const uint32_t* r2, const uint32_t* r3, double* out,           \
                                 int* ebin, int* abin, hipStream_t st);                                             \

@@ -784,6 +784,62 @@ __global__ void k_sample(Tables<R> T, int kind, int n, const double* e, const in
   abin[i] = ab;
 }
 
+// Final states of single interactions of a primary of energy E moving along
+// +z in material imat, one Philox stream per sample (key = seed, history =
+// sample index): the GPU form of the reference's model tests
+// (testAPI.cc:92-199). out_v / out_c per kind:
+//   0 PerformBrem   (SimDPMLike.cc:661-695): photon energy, photon cos(theta)
+//   1 PerformMoller (SimDPMLike.cc:699-728): secondary e- energy, its cos(theta)
+//   2 PerformMSCAngularDeflection (SimDPMLike.cc:731-750): 1-cos(theta), cos(theta)
+//   3 Compton       (SimDPMLike.cc:512-570): scattered photon energy, its cos(theta)
+__global__ void k_perform(Tables<R> T, int kind, int n, double energy, int imat, uint32_t k0, uint32_t k1,
+                          double* out_v, double* out_c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Rng rng;
+  rng.k0 = k0; rng.k1 = k1;
+  rng.hist = (uint32_t)i;
+  rng.slo = 0; rng.shi = 0; rng.ndraw = 0; rng.nchild = 0; rng.nblk = 0;
+  rng.topup();
+  const R E = (R)energy;
+  R v = (R)0, cost = (R)1, sint = (R)0;
+  double omu_out = 0.0;
+  if (kind == 0) {
+    const uint32_t q1 = rng.draw24(), q2 = rng.draw24(), q3 = rng.draw24();
+    v = sample_sb(T.sb, E, imat, q1, q2, q3, (int*)nullptr, (int*)nullptr);
+    const R dum0 = (R)(kEMC2 * 0.7071067812) / (E + (R)kEMC2);
+    cost = rmax((R)-1.0, rmin((R)1.0, (R)1.0 - dum0 * dum0));
+    sint = rsqrt_(((R)1.0 + cost) * ((R)1.0 - cost));
+  } else if (kind == 1) {
+    const uint32_t q1 = rng.draw24(), q2 = rng.draw24(), q3 = rng.draw24();
+    v = sample_moller(T.mo, E, q1, q2, q3, (int*)nullptr, (int*)nullptr);
+    const R k2EMC2 = (R)(2.0 * kEMC2);
+    cost = rmin((R)1.0, rsqrt_(v * (E + k2EMC2) / (E * (v + k2EMC2))));
+    sint = rsqrt_(((R)1.0 + cost) * ((R)1.0 - cost));
+  } else if (kind == 2) {
+    const uint32_t q1 = rng.draw24(), q2 = rng.draw24();
+    const R omu = sample_gs_omu(T.gs, E, q1, q2, (int*)nullptr, (int*)nullptr);
+    const R o = rmax((R)0.0, rmin((R)2.0, omu));
+    cost = (R)1.0 - o;
+    sint = rsqrt_(o * ((R)2.0 - o));
+    omu_out = (double)o;
+  } else {
+    const uint32_t q1 = rng.draw24(), q2 = rng.draw24(), q3 = rng.draw24();
+    const R eps = sample_kn(T.kn, E, q1, q2, q3, (int*)nullptr, (int*)nullptr);
+    cost = (R)1.0 - ((R)1.0 - eps) / (eps * E * (R)(1.0 / kEMC2));
+    v = eps * E;
+    sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
+  }
+  // azimuth and rotation to the frame of a primary along +z (the identity)
+  const R phi = (R)(2.0 * kPI) * rng.uni<R>();
+  R sp, cp;
+  rsincos(phi, &sp, &cp);
+  R du = sint * cp, dv = sint * sp, dw = cost;
+  rotate_to_lab(du, dv, dw, (R)0, (R)0, (R)1);
+  out_v[i] = kind == 2 ? omu_out : (double)v;
+  out_c[i] = (double)dw;
+}
+
 __global__ void k_eval(Tables<R> T, int which, int n, const double* e, const int* imat, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -833,6 +889,14 @@ hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, const SourceP
     hipLaunchKernelGGL(k_electrons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   else
     hipLaunchKernelGGL(k_electrons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+  return hipGetLastError();
+}
+
+hipError_t DPM_CAT(launch_perform, DPM_SFX)(const Tables<R>& T, int kind, int n, double energy, int imat,
+                                            uint64_t seed, double* out_v, double* out_c, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_perform, dim3((n + 255) / 256), dim3(256), 0, st, T, kind, n, energy, imat, (uint32_t)seed,
+                     (uint32_t)(seed >> 32), out_v, out_c);
   return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "second
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
-            "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_last_error", "dpm_abi_version")
+            "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_last_error", "dpm_abi_version")
 
 
 class DpmError(RuntimeError):
@@ -103,6 +103,7 @@ def lib():
         L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
         L.dpm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
         L.dpm_tally_fetch.argtypes = [vp, vp, vp]
+        L.dpm_perform.argtypes = [vp, ip, ip, ip, dp, ip, ctypes.c_uint64, vp, vp]
         L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ctypes.POINTER(Schedule)]
         L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
                                             ctypes.POINTER(Schedule)]
@@ -154,6 +155,16 @@ class Tables:
         _check(lib().dpm_sample(self.h, k, precision, n, _ptr(e), _ptr(im), _ptr(u1), _ptr(u2), _ptr(u3), _ptr(out),
                                 _ptr(eb), _ptr(ab)))
         return out, eb, ab
+
+    def perform(self, kind, n, energy, imat=0, seed=1, precision=DPM_FP32):
+        """n single interactions of a primary along +z (dpm_perform); kind in
+        brem|moller|msc|compton. Returns (value, cos): photon/secondary energy,
+        or 1-cos(theta) for msc, and cos(theta)."""
+        k = {"brem": 0, "moller": 1, "msc": 2, "compton": 3}[kind]
+        v = np.zeros(n)
+        c = np.zeros(n)
+        _check(lib().dpm_perform(self.h, k, precision, n, energy, imat, seed, _ptr(v), _ptr(c)))
+        return v, c
 
     def eval(self, which, energy, imat=None, precision=DPM_FP32):
         e = np.ascontiguousarray(energy, dtype=np.float64)

@@ -255,6 +255,20 @@ int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* ener
                const uint32_t* r1, const uint32_t* r2, const uint32_t* r3,
                double* out, int32_t* ebin, int32_t* abin);
 
+/* Final states of `n` single interactions of a primary of `energy` moving
+ * along +z in material `imat` (used by kind 0 only), each with its own
+ * Philox stream (key `seed`, history = sample index): the device form of the
+ * reference's model tests BremTest / MollerTest / MSCAngularDeflectionTest
+ * (testAPI.cc:92-199), which drive PerformBrem (SimDPMLike.cc:661),
+ * PerformMoller (:699) and PerformMSCAngularDeflection (:731); kind 3 is the
+ * Compton final state of KeepTrackingPhoton (:512-570).
+ *   kind 0 brem:    out_value = photon energy,      out_cos = photon cos(theta)
+ *   kind 1 Moller:  out_value = secondary energy,   out_cos = secondary cos(theta)
+ *   kind 2 MSC:     out_value = 1 - cos(theta),     out_cos = cos(theta)
+ *   kind 3 Compton: out_value = photon energy,      out_cos = photon cos(theta) */
+int dpm_perform(dpm_tables* t, int kind, int precision, int n, double energy, int imat, uint64_t seed,
+                double* out_value, double* out_cos);
+
 /* The interpolations SimDPMLike.cc uses, on the device:
  *   which 0 ITr1(E,m) 1 dE/dx(E,m) 2 brem IMFP(E,m) 3 Moller IMFP(E) 4 K1(E)
  *         5 max total IMFP(E) 6 total(E,m) 7 Compton(E,m) 8 pair(E,m) */

@@ -1136,6 +1136,22 @@ double oracle_sample_gs(double e, double r1, double r2, int* ebin, int* cbin) {
   return sample_gs(e, r1, r2, ebin, cbin);
 }
 
+// n samples at one energy with explicit FP64 uniforms (kind 0 SB, 1 Moller,
+// 2 KN, 3 GS: returns cos(theta)); for histograms of the model tests
+int oracle_sample_many(int kind, int n, double e, int imat, const double* r1, const double* r2, const double* r3,
+                       double* out) {
+  for (int i = 0; i < n; ++i) {
+    switch (kind) {
+      case 0: out[i] = sample_sb(e, imat, r1[i], r2[i], r3[i], nullptr, nullptr); break;
+      case 1: out[i] = sample_moller(e, r1[i], r2[i], r3[i], nullptr, nullptr); break;
+      case 2: out[i] = sample_kn(e, r1[i], r2[i], r3[i], nullptr, nullptr); break;
+      case 3: out[i] = sample_gs(e, r1[i], r2[i], nullptr, nullptr); break;
+      default: g_err = "unknown sampler"; return -1;
+    }
+  }
+  return 0;
+}
+
 // which: 0 itr1(E,m) 1 dedx(E,m) 2 brem(E,m) 3 moller(E) 4 K1(E) 5 gmax(E)
 //        6 total(E,m) 7 compton(E,m) 8 pair(E,m)
 double oracle_eval(int which, double e, int imat) {

@@ -15,6 +15,7 @@ sources with `make -C oracle ref` into oracle/_ref/, never copying them):
 import os
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -32,7 +33,7 @@ def lattice(rng, n):
 
 
 def main():
-    datadir = os.path.join(ROOT, "_work", "golden_tables_cfg3")
+    datadir = tempfile.mkdtemp(prefix="dpm_golden_cfg3_")
     dpm_tablegen.generate(datadir, 3, verbose=False)
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     probe = os.path.join(ROOT, "oracle", "_ref", "ref_probe")

@@ -65,6 +65,9 @@ def lib():
             f.argtypes = args + [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_sample_gs.restype = ctypes.c_double
         L.oracle_sample_gs.argtypes = [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2
+        dptr = ctypes.POINTER(ctypes.c_double)
+        L.oracle_sample_many.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dptr, dptr, dptr,
+                                         dptr]
         L.oracle_eval.restype = ctypes.c_double
         L.oracle_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.oracle_philox.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
@@ -123,6 +126,19 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
     return dict(edep=edep.reshape(mat.shape), batch_sum=bs, batch_sum2=bs2, hist_edep=hed,
                 e_escaped=out.e_escaped, e_scored=out.e_scored, n_pair=out.n_pair, n_annih=out.n_annih,
                 counters=dict(zip(COUNTER_NAMES, list(out.cnt))))
+
+
+def sample_many(kind, e, r, imat=0):
+    """FP64 reference samples at energy e for uniforms r of shape (n, 3);
+    kind in sb|mo|kn|gs (gs returns cos(theta))."""
+    k = {"sb": 0, "mo": 1, "kn": 2, "gs": 3}[kind]
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    cols = [np.ascontiguousarray(r[:, j]) for j in range(3)]
+    out = np.zeros(r.shape[0])
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    if lib().oracle_sample_many(k, r.shape[0], e, imat, dp(cols[0]), dp(cols[1]), dp(cols[2]), dp(out)) != 0:
+        raise RuntimeError(lib().oracle_error().decode())
+    return out
 
 
 def sample(kind, e, r, imat=0):
