@@ -13,6 +13,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "lib", "libdpm_amd.so")
 CLI_PATH = os.path.join(PKG, "bin", "dpm_Simulate")
+VOXEL_CLI_PATH = os.path.join(PKG, "bin", "dpm_VoxelDose")
 
 DPM_FP32, DPM_FP64 = 0, 1
 ELECTRON, PHOTON, POSITRON = -1, 0, 1
@@ -319,3 +320,36 @@ def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, pre
     val = np.zeros(nnz)
     _check(lib().dpm_beamlets_fetch(tables.h, _ptr(col), _ptr(val)))
     return row_ptr, col, val, st.as_dict()
+
+
+def read_voxel_dose(path):
+    """A voxel-dose file of dpm_VoxelDose (host/DoseIO.hh): header, then dose
+    [Gy per history] and relative standard deviation, each [nz][ny][nx] float64."""
+    with open(path, "rb") as f:
+        if not f.readline().startswith(b"# dpm-amd voxel dose v1"):
+            raise ValueError("not a dpm-amd voxel dose file: %s" % path)
+        nx, ny, nz, lbox, nhist, nbatch = f.readline().split()
+        nx, ny, nz = int(nx), int(ny), int(nz)
+        n = nx * ny * nz
+        data = np.frombuffer(f.read(16 * n), dtype="<f8")
+    if data.size != 2 * n:
+        raise ValueError("truncated voxel dose file: %s" % path)
+    shape = (nz, ny, nx)
+    return dict(nx=nx, ny=ny, nz=nz, lbox=float(lbox), nhist=int(nhist), nbatch=int(nbatch),
+                dose=data[:n].reshape(shape), rel_sigma=data[n:].reshape(shape))
+
+
+def read_csr(path):
+    """A dose-influence matrix of dpm_VoxelDose -B (host/DoseIO.hh): returns
+    (row_ptr int64[nb+1], col int32[nnz], val float64[nnz] in MeV, nvox)."""
+    with open(path, "rb") as f:
+        if not f.readline().startswith(b"# dpm-amd csr v1"):
+            raise ValueError("not a dpm-amd CSR file: %s" % path)
+        nb, nvox, nnz = (int(x) for x in f.readline().split())
+        row_ptr = np.frombuffer(f.read(8 * (nb + 1)), dtype="<i8")
+        col = np.frombuffer(f.read(4 * nnz), dtype="<i4")
+        val = np.frombuffer(f.read(8 * nnz), dtype="<f8")
+    if row_ptr.size != nb + 1 or col.size != nnz or val.size != nnz:
+        raise ValueError("truncated CSR file: %s" % path)
+    return row_ptr, col, val, nvox
+

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import dpm_amd
@@ -62,3 +63,33 @@ def test_null_arguments_are_rejected():
     L = dpm_amd.lib()
     assert L.dpm_simulate(None, None, None, 0, None, None, None, None) == -1
     assert L.dpm_sample(None, 0, 0, 0, None, None, None, None, None, None, None, None) == -1
+
+
+def test_voxeldose_cli_help():
+    out = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-h"], capture_output=True, text=True)
+    assert out.returncode == 0
+    for flag in ("-d", "-x", "-m", "-r", "-E", "-F", "-k", "-B", "-o"):
+        assert "  %s " % flag in out.stdout
+    bad = subprocess.run([dpm_amd.VOXEL_CLI_PATH], capture_output=True, text=True)
+    assert bad.returncode != 0
+
+
+def test_dose_file_readers_round_trip(tmp_path):
+    nx, ny, nz = 3, 4, 5
+    rng = np.random.default_rng(0)
+    dose, rel = rng.random((nz, ny, nx)), rng.random((nz, ny, nx))
+    p = tmp_path / "d.bin"
+    p.write_bytes(b"# dpm-amd voxel dose v1\n3 4 5 2 1000 10\n" + dose.tobytes() + rel.tobytes())
+    d = dpm_amd.read_voxel_dose(str(p))
+    assert (d["nx"], d["ny"], d["nz"], d["lbox"], d["nhist"], d["nbatch"]) == (3, 4, 5, 2.0, 1000, 10)
+    assert np.array_equal(d["dose"], dose) and np.array_equal(d["rel_sigma"], rel)
+    p.write_bytes(b"# dpm-amd voxel dose v1\n3 4 5 2 1000 10\n" + dose.tobytes())
+    with pytest.raises(ValueError):
+        dpm_amd.read_voxel_dose(str(p))
+    rp = np.array([0, 2, 3], np.int64)
+    col = np.array([1, 7, 4], np.int32)
+    val = np.array([0.5, 1.5, 2.5])
+    q = tmp_path / "m.csr"
+    q.write_bytes(b"# dpm-amd csr v1\n2 60 3\n" + rp.tobytes() + col.tobytes() + val.tobytes())
+    r2, c2, v2, nvox = dpm_amd.read_csr(str(q))
+    assert nvox == 60 and np.array_equal(r2, rp) and np.array_equal(c2, col) and np.array_equal(v2, val)
