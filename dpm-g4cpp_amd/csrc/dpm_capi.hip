@@ -585,10 +585,13 @@ struct HostEntry {
   double x, y, e;
   int particle;
   int row;          // tally row, or -1 (batch row / row 0)
+  double angle_deg = 0.0;   // gantry fields
 };
 struct HostSource {
   int kind = DPM_SRC_PENCIL;
   double emin = 0, emax = 0, fx = 0, fy = 0;
+  int spectrum = DPM_SPEC_FLAT;
+  double emode = 0;
   std::vector<HostEntry> list[2];   // [0] photons, [1] e-/e+
   void add(const HostEntry& h) { list[h.particle == DPM_PHOTON ? 0 : 1].push_back(h); }
   uint64_t count(int k) const {
@@ -601,14 +604,29 @@ struct HostSource {
 // Uploads the source into one device block: SourceParams followed by the
 // per-list arrays (cum, first, x, y, e, particle, row); returns the device
 // address of the SourceParams.
+// cos and sin of a gantry angle in degrees, exact at multiples of 90
+void gantry_cos_sin(double deg, double* c, double* s) {
+  const double q = deg / 90.0;
+  if (q == std::floor(q)) {
+    const int k = ((int)std::fmod(q, 4.0) + 4) % 4;
+    const double cs[4] = {1.0, 0.0, -1.0, 0.0}, sn[4] = {0.0, 1.0, 0.0, -1.0};
+    *c = cs[k];
+    *s = sn[k];
+    return;
+  }
+  const double a = deg * (3.14159265358979323846 / 180.0);
+  *c = std::cos(a);
+  *s = std::sin(a);
+}
+
 int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, uint64_t hist_begin, uint64_t nhist,
                   SourceParams** out) {
   size_t bytes = (sizeof(SourceParams) + 255) & ~(size_t)255;
-  size_t off[2][7];
+  size_t off[2][9];
   for (int k = 0; k < 2; ++k) {
     const size_t n = hs.list[k].size();
-    const size_t sz[7] = {8 * (n + 1), 8 * n, 8 * n, 8 * n, 8 * n, 4 * n, 4 * n};
-    for (int a = 0; a < 7; ++a) {
+    const size_t sz[9] = {8 * (n + 1), 8 * n, 8 * n, 8 * n, 8 * n, 4 * n, 4 * n, 8 * n, 8 * n};
+    for (int a = 0; a < 9; ++a) {
       off[k][a] = bytes;
       bytes += (sz[a] + 255) & ~(size_t)255;
     }
@@ -622,6 +640,8 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   S.nbatch = nbatch;
   S.emin = hs.emin;
   S.emax = hs.emax;
+  S.spectrum = hs.spectrum;
+  S.emode = hs.emode;
   S.fx = hs.fx;
   S.fy = hs.fy;
   S.z0 = z0;
@@ -632,7 +652,9 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     // several entries (beamlets): interleave their histories with a
     // multiplicative bijection of the index (the prime must not divide the count)
     S.perm[k] = 0;
-    if (hs.list[k].size() > 1 && S.count[k] > 1)
+    bool rows = false;
+    for (const HostEntry& h : hs.list[k]) rows = rows || h.row >= 0;
+    if (rows && hs.list[k].size() > 1 && S.count[k] > 1)
       for (uint64_t prime : {2654435761ull, 4294967291ull, 2246822519ull})
         if (S.count[k] % prime != 0) { S.perm[k] = prime; break; }
   }
@@ -646,6 +668,8 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     double* e = (double*)(h.data() + off[k][4]);
     int* pt = (int*)(h.data() + off[k][5]);
     int* row = (int*)(h.data() + off[k][6]);
+    double* ca = (double*)(h.data() + off[k][7]);
+    double* sa = (double*)(h.data() + off[k][8]);
     cum[0] = 0;
     for (int i = 0; i < n; ++i) {
       cum[i + 1] = cum[i] + L[i].n;
@@ -653,6 +677,7 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
       x[i] = L[i].x; y[i] = L[i].y; e[i] = L[i].e;
       pt[i] = L[i].particle;
       row[i] = L[i].row;
+      gantry_cos_sin(L[i].angle_deg, &ca[i], &sa[i]);
     }
     PrimList& D = S.list[k];
     D.n = n;
@@ -663,6 +688,8 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     D.e = (const double*)(base + off[k][4]);
     D.particle = (const int*)(base + off[k][5]);
     D.row = (const int*)(base + off[k][6]);
+    D.ca = (const double*)(base + off[k][7]);
+    D.sa = (const double*)(base + off[k][8]);
   }
   std::memcpy(h.data(), &S, sizeof(S));
   // synchronous: the host staging vector goes out of scope on return
@@ -707,8 +734,11 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
   // batch rows need one source entry (history index = history id - begin)
-  const size_t nent = hs.list[0].size() + hs.list[1].size();
-  const bool single = nent == 1 && (hs.list[0].empty() ? hs.list[1][0] : hs.list[0][0]).row < 0;
+  // batch rows follow history ids: one particle kind (one list, entries in id
+  // order) and no per-entry rows
+  bool single = hs.list[0].empty() != hs.list[1].empty();
+  for (int k = 0; k < 2; ++k)
+    for (const HostEntry& h : hs.list[k]) single = single && h.row < 0;
   const int nbatch = (single && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
   int64_t chunk = opt->chunk > 0 ? opt->chunk : kDefaultChunk;
   if (chunk > ((int64_t)1 << 24)) chunk = (int64_t)1 << 24;
@@ -887,8 +917,29 @@ HostSource source_from(const dpm_source* src, uint64_t nhist, uint64_t hist_begi
   hs.emax = src->emax;
   hs.fx = src->field_x;
   hs.fy = src->field_y;
-  hs.add(HostEntry{hist_begin, nhist, src->x0, src->y0, src->energy, src->particle, -1});
+  hs.spectrum = src->spectrum;
+  hs.emode = src->emode;
+  HostEntry h{hist_begin, nhist, src->x0, src->y0, src->energy, src->particle, -1};
+  h.angle_deg = src->angle_deg;
+  hs.add(h);
   return hs;
+}
+
+int check_source(const dpm_tables* t, int particle, double e, double emin, double emax);
+
+// the source description of dpm_simulate / dpm_simulate_fields
+int validate_source(const dpm_tables* t, const dpm_phantom* p, const dpm_source* src) {
+  if (src->kind != DPM_SRC_PENCIL && src->kind != DPM_SRC_FIELD && src->kind != DPM_SRC_GANTRY)
+    return set_error(DPM_E_INVALID, "unknown source kind");
+  if (src->kind == DPM_SRC_GANTRY && p->geom != GEOM_VOXEL)
+    return set_error(DPM_E_INVALID, "gantry fields need a voxel phantom");
+  if (!std::isfinite(src->angle_deg)) return set_error(DPM_E_INVALID, "gantry angle is not finite");
+  if (src->spectrum != DPM_SPEC_FLAT && src->spectrum != DPM_SPEC_TRIANGULAR)
+    return set_error(DPM_E_INVALID, "unknown spectrum");
+  if (src->spectrum == DPM_SPEC_TRIANGULAR && !(src->emax > src->emin && src->emode >= src->emin &&
+                                                src->emode <= src->emax))
+    return set_error(DPM_E_INVALID, "triangular spectrum needs emin <= emode <= emax, emin < emax");
+  return check_source(t, src->particle, src->energy, src->emin, src->emax);
 }
 
 int check_source(const dpm_tables* t, int particle, double e, double emin, double emax) {
@@ -1049,6 +1100,9 @@ void dpm_phantom_destroy(dpm_phantom* p) {
 
 int64_t dpm_phantom_num_bins(const dpm_phantom* p) { return p ? (p->geom == GEOM_SLAB ? p->nbins : p->nvox) : 0; }
 
+static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, uint64_t nhist,
+                           const dpm_run_options& opt, double* tally, double* steps, dpm_stats* st);
+
 int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
                  const dpm_run_options* opt_in, double* tally, double* steps, dpm_stats* st) {
   if (!t || !p || !src) return set_error(DPM_E_INVALID, "null argument");
@@ -1060,14 +1114,45 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   if (opt_in) opt = *opt_in;
   if (opt.hist_begin + nhist > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
   if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
-  if (src->kind != DPM_SRC_PENCIL && src->kind != DPM_SRC_FIELD) return set_error(DPM_E_INVALID, "unknown source kind");
   if (opt.nbatch > 1 && (uint64_t)opt.nbatch > nhist) return set_error(DPM_E_INVALID, "more batches than histories");
-  int rc = check_source(t, src->particle, src->energy, src->emin, src->emax);
+  const int rc = validate_source(t, p, src);
   if (rc != DPM_OK) return rc;
+  return simulate_source(t, p, source_from(src, nhist, opt.hist_begin), nhist, opt, tally, steps, st);
+}
+
+int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, const dpm_field* fields, int nf,
+                        const dpm_run_options* opt_in, double* tally, double* steps, dpm_stats* st) {
+  if (!t || !p || !src || !fields || nf <= 0) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (nf > (1 << 20)) return set_error(DPM_E_INVALID, "too many fields");
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  if (opt_in) opt = *opt_in;
+  if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  int rc = validate_source(t, p, src);
+  if (rc != DPM_OK) return rc;
+  HostSource hs = source_from(src, 1, 0);
+  hs.list[0].clear();
+  hs.list[1].clear();
+  uint64_t total = 0;
+  for (int i = 0; i < nf; ++i) {
+    if (fields[i].nhist == 0) return set_error(DPM_E_INVALID, "field without histories");
+    if (!std::isfinite(fields[i].angle_deg)) return set_error(DPM_E_INVALID, "gantry angle is not finite");
+    HostEntry h{opt.hist_begin + total, fields[i].nhist, fields[i].x0, fields[i].y0, src->energy, src->particle, -1};
+    h.angle_deg = fields[i].angle_deg;
+    hs.add(h);
+    total += fields[i].nhist;
+  }
+  if (opt.hist_begin + total > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  if (opt.nbatch > 1 && (uint64_t)opt.nbatch > total) return set_error(DPM_E_INVALID, "more batches than histories");
+  return simulate_source(t, p, hs, total, opt, tally, steps, st);
+}
+
+static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, uint64_t nhist,
+                           const dpm_run_options& opt, double* tally, double* steps, dpm_stats* st) {
+  int rc = DPM_OK;
   HIP_TRY(hipSetDevice(t->device));
   const hipStream_t s = t->stream;
-  const HostSource hs = source_from(src, nhist, opt.hist_begin);
-
   const bool slab = p->geom == GEOM_SLAB;
   const int nbatch = (!slab && opt.nbatch > 1) ? opt.nbatch : 0;
   const int64_t nb = slab ? p->nbins : p->nvox;
@@ -1300,8 +1385,7 @@ int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
   if (nsample < 1000 || nsample > ((uint64_t)1 << 24)) return set_error(DPM_E_INVALID, "sample of 1e3 .. 2^24 histories");
   if (precision != DPM_FP32 && precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
-  if (src->kind != DPM_SRC_PENCIL && src->kind != DPM_SRC_FIELD) return set_error(DPM_E_INVALID, "unknown source kind");
-  int rc = check_source(t, src->particle, src->energy, src->emin, src->emax);
+  int rc = validate_source(t, p, src);
   if (rc != DPM_OK) return rc;
   HIP_TRY(hipSetDevice(t->device));
   const HostSource hs = source_from(src, nsample, 0);

@@ -67,13 +67,17 @@ struct PrimList {
   const double* e;          // [n] energy (MeV) unless a spectrum is sampled
   const int* particle;      // [n] -1 e-, 0 gamma, +1 e+
   const int* row;           // [n] tally row, or -1: batch row (or row 0)
+  const double* ca;         // [n] gantry fields: cos and sin of the angle
+  const double* sa;
 };
 
 // Source of one run, resident in device memory for its duration
 struct SourceParams {
-  int kind;                 // 0 pencil, 1 rectangular field around the entry point
+  int kind;                 // 0 pencil, 1 rectangular field around the entry point, 2 gantry field
   int nbatch;               // > 1: tally row = batch index - first batch of the chunk
-  double emin, emax;        // emax > emin: energy uniform in [emin, emax)
+  double emin, emax;        // emax > emin: energy from the spectrum over [emin, emax)
+  int spectrum;             // 0 flat, 1 triangular with mode emode
+  double emode;
   double fx, fy, z0;
   uint64_t hist_begin, nhist_total;
   PrimList list[2];         // [0] photons, [1] electrons and positrons

@@ -295,7 +295,46 @@ __device__ __forceinline__ void annihilate(BlockAcc& a, const KParams<R>& P, Tra
 // the source draws (field position x, y; spectrum energy) from the history's
 // own stream, in that order
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
+// Gantry field (voxel phantom): the field point (fu, fv) lies on the plane
+// through the phantom centre perpendicular to the beam direction
+// d = (sa, 0, ca), with axes a = (ca, 0, -sa) and b = (0, 1, 0); the particle
+// starts where the ray of direction d through it enters the phantom (slab
+// method), 1e-3 mm inside. False if the ray misses the phantom.
+__device__ __forceinline__ bool gantry_entry(R fu, R fv, R ca, R sa, const Geometry<R>& G, R pos[3], R dir[3]) {
+  const R q[3] = {fu * ca, fv, -fu * sa};
+  const R d[3] = {sa, (R)0, ca};
+  const R h[3] = {-G.org[0], -G.org[1], -G.org[2]};
+  R tin = (R)-1e30, tout = (R)1e30;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] == (R)0) {
+      if (q[i] <= -h[i] || q[i] >= h[i]) return false;
+      continue;
+    }
+    const R t1 = (-h[i] - q[i]) / d[i], t2 = (h[i] - q[i]) / d[i];
+    tin = rmax(tin, rmin(t1, t2));
+    tout = rmin(tout, rmax(t1, t2));
+  }
+  if (!(tin + (R)2e-3 < tout)) return false;
+  const R t = tin + (R)1e-3;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pos[i] = q[i] + t * d[i];
+    dir[i] = d[i];
+  }
+  return true;
+}
+
+// inverse CDF of the triangular distribution (lo, mode, hi)
+__device__ __forceinline__ R triangular(R u, R lo, R mode, R hi) {
+  const R fc = (mode - lo) / (hi - lo);
+  if (u < fc) return lo + rsqrt_(u * (hi - lo) * (mode - lo));
+  return hi - rsqrt_(((R)1.0 - u) * (hi - lo) * (hi - mode));
+}
+
+// returns false when a gantry ray misses the phantom (t.E then holds the
+// primary's energy, which the caller counts as escaped)
+__device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
                                             const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
                                             double& eprim, int k) {
   // beamlets: neighbouring lanes take histories of different entries, so that
@@ -317,14 +356,19 @@ __device__ __forceinline__ void gen_primary(const KParams<R>& P, const SourcePar
   t.pos[0] = (R)x0;
   t.pos[1] = (R)y0;
   t.pos[2] = (R)S->z0;
-  if (S->kind == 1) {
+  if (S->kind >= 1) {
     const R ux = t.rng.uni<R>(), uy = t.rng.uni<R>();
     t.pos[0] = (R)x0 + (ux - (R)0.5) * (R)S->fx;
     t.pos[1] = (R)y0 + (uy - (R)0.5) * (R)S->fy;
   }
   R E = (R)L.e[lo];
-  if (S->emax > S->emin) E = (R)S->emin + t.rng.uni<R>() * (R)(S->emax - S->emin);
+  if (S->emax > S->emin)
+    E = S->spectrum == 1 ? triangular(t.rng.uni<R>(), (R)S->emin, (R)S->emode, (R)S->emax)
+                         : (R)S->emin + t.rng.uni<R>() * (R)(S->emax - S->emin);
   eprim += (double)E;
+  t.dir[0] = (R)0; t.dir[1] = (R)0; t.dir[2] = (R)1;
+  bool inside = true;
+  if (S->kind == 2) inside = gantry_entry(t.pos[0], t.pos[1], (R)L.ca[lo], (R)L.sa[lo], P.G, t.pos, t.dir);
   int row = L.row[lo];
   if (row < 0)
     row = S->nbatch > 1
@@ -334,8 +378,8 @@ __device__ __forceinline__ void gen_primary(const KParams<R>& P, const SourcePar
   t.type = pt == 0 ? 0 : (pt < 0 ? 1 : 2);
   t.tag = (uint32_t)row;
   t.iter = 0;
-  t.dir[0] = (R)0; t.dir[1] = (R)0; t.dir[2] = (R)1;
   t.E = E;
+  return inside;
 }
 
 // wave sum of the lanes' primary energies into the block accumulator
@@ -502,7 +546,10 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
-        else gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 0);
+        else if (!gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 0)) {
+          escape(acc, t.E);
+          return false;
+        }
         t.rng.topup();
         return start_track<GEOM>(acc, P, t);
       });
@@ -744,7 +791,10 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
         if (idx < n_in) load_track(in, idx, P, t);
-        else gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 1);
+        else if (!gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 1)) {
+          escape(acc, t.E);
+          return false;
+        }
         t.rng.topup();
         return electron_init<GEOM>(acc, P, t, s);
       });

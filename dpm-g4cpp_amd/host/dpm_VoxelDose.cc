@@ -9,6 +9,7 @@
 //   -r densities (raw f32, same order; default: nominal density of each voxel's material)
 //   -p particle (e-, gamma, e+) -e energy [MeV]  or  -E emin,emax (uniform spectrum)
 //   -F field side [mm] (0: pencil)  -X -Y entry point [mm]
+//   -A gantry angle [deg] (field through the phantom centre)  -T mode of a triangular spectrum (with -E)
 //   -n histories  -k batches  -s seed  -f FP64 transport  -g device  -o output
 //   -B beamlet list (lines "x y particle energy histories"; writes CSR)  -t CSR threshold
 #include <getopt.h>
@@ -33,6 +34,8 @@ struct Args {
   std::string data = "./data", mat_file, rho_file, beamlets, out = "dose.bin", particle = "gamma";
   int nx = 0, ny = 0, nz = 0, nbatch = 10, device = 0;
   double lbox = 2.0, energy = 10.0, emin = 0.0, emax = 0.0, field = 0.0, x0 = 0.0, y0 = 0.0, threshold = 1e-3;
+  double angle = 0.0, emode = 0.0;
+  bool gantry = false, triangular = false;
   uint64_t nhist = 100000, seed = 1;
   bool fp64 = false;
 };
@@ -49,6 +52,8 @@ void Help() {
                "  -E lo,hi   uniform energy spectrum instead of -e\n"
                "  -F mm      square field side, 0 = pencil            (default 0)\n"
                "  -X -Y mm   entry point                              (default 0 0)\n"
+               "  -A deg     gantry angle: beam (sin a, 0, cos a) through the phantom centre\n"
+               "  -T MeV     mode of a triangular spectrum over -E lo,hi\n"
                "  -n N       histories                                (default 1e5)\n"
                "  -k K       history batches for the uncertainty      (default 10)\n"
                "  -s seed    Philox seed                              (default 1)\n"
@@ -91,7 +96,7 @@ void Check(int rc, const char* what) {
 int main(int argc, char* argv[]) {
   Args a;
   int c;
-  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:n:k:s:fg:B:t:o:")) != -1) {
+  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:n:k:s:fg:B:t:o:")) != -1) {
     switch (c) {
       case 'd': a.data = optarg; break;
       case 'x': a.nx = std::atoi(optarg); break;
@@ -112,6 +117,8 @@ int main(int argc, char* argv[]) {
       case 'F': a.field = std::atof(optarg); break;
       case 'X': a.x0 = std::atof(optarg); break;
       case 'Y': a.y0 = std::atof(optarg); break;
+      case 'A': a.angle = std::atof(optarg); a.gantry = true; break;
+      case 'T': a.emode = std::atof(optarg); a.triangular = true; break;
       case 'n': a.nhist = (uint64_t)std::atof(optarg); break;
       case 'k': a.nbatch = std::atoi(optarg); break;
       case 's': a.seed = (uint64_t)std::atoll(optarg); break;
@@ -196,7 +203,10 @@ int main(int argc, char* argv[]) {
               << a.out << std::endl;
   } else {
     dpm_source src = {};
-    src.kind = a.field > 0 ? DPM_SRC_FIELD : DPM_SRC_PENCIL;
+    src.kind = a.gantry ? DPM_SRC_GANTRY : (a.field > 0 ? DPM_SRC_FIELD : DPM_SRC_PENCIL);
+    src.angle_deg = a.angle;
+    src.spectrum = a.triangular ? DPM_SPEC_TRIANGULAR : DPM_SPEC_FLAT;
+    src.emode = a.emode;
     src.particle = ParticleCode(a.particle);
     src.energy = a.energy;
     src.emin = a.emin;

@@ -17,14 +17,16 @@ VOXEL_CLI_PATH = os.path.join(PKG, "bin", "dpm_VoxelDose")
 
 DPM_FP32, DPM_FP64 = 0, 1
 ELECTRON, PHOTON, POSITRON = -1, 0, 1
-SRC_PENCIL, SRC_FIELD = 0, 1
+SRC_PENCIL, SRC_FIELD, SRC_GANTRY = 0, 1, 2
+SPEC_FLAT, SPEC_TRIANGULAR = 0, 1
 COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries", "e_steps", "g_steps",
                  "tally_atomics")
 
 EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_tables_num_materials",
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
-            "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_last_error", "dpm_abi_version")
+            "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_simulate_fields", "dpm_last_error",
+            "dpm_abi_version")
 
 
 class DpmError(RuntimeError):
@@ -34,7 +36,13 @@ class DpmError(RuntimeError):
 class Source(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("particle", ctypes.c_int), ("energy", ctypes.c_double),
                 ("emin", ctypes.c_double), ("emax", ctypes.c_double), ("x0", ctypes.c_double),
-                ("y0", ctypes.c_double), ("field_x", ctypes.c_double), ("field_y", ctypes.c_double)]
+                ("y0", ctypes.c_double), ("field_x", ctypes.c_double), ("field_y", ctypes.c_double),
+                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double)]
+
+
+class Field(ctypes.Structure):
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("angle_deg", ctypes.c_double),
+                ("nhist", ctypes.c_uint64)]
 
 
 class RunOptions(ctypes.Structure):
@@ -104,6 +112,8 @@ def lib():
         L.dpm_eval.argtypes = [vp, ip, ip, ip, vp, vp, vp]
         L.dpm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
         L.dpm_tally_fetch.argtypes = [vp, vp, vp]
+        L.dpm_simulate_fields.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.POINTER(Field), ip,
+                                          ctypes.POINTER(RunOptions), vp, vp, ctypes.POINTER(Stats)]
         L.dpm_perform.argtypes = [vp, ip, ip, ip, dp, ip, ctypes.c_uint64, vp, vp]
         L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ctypes.POINTER(Schedule)]
         L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
@@ -235,15 +245,17 @@ def pinned_empty(n, dtype=np.float64):
 
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
-             out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True):
+             out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
+             angle=None, emode=None):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
     phantom.nbins elements to write into (e.g. from pinned_empty), returned
     reshaped to the phantom's shape. fetch=False leaves the result on the
-    device (tally and steps are returned as None; see tally_fetch)."""
-    src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
-                 x0=x0, y0=y0, field_x=field, field_y=field)
+    device (tally and steps are returned as None; see tally_fetch). angle
+    (degrees) selects the gantry source (voxel phantoms), emode the triangular
+    spectrum (emin, emode, emax)."""
+    src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode)
     he = np.zeros(nhist) if per_history else None
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
@@ -272,6 +284,37 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     return tally, steps, st.as_dict(), he
 
 
+def _source(particle, energy, emin, emax, field, x0, y0, angle, emode):
+    kind = SRC_GANTRY if angle is not None else (SRC_FIELD if field > 0 else SRC_PENCIL)
+    return Source(kind=kind, particle=particle, energy=energy, emin=emin, emax=emax, x0=x0, y0=y0, field_x=field,
+                  field_y=field, angle_deg=0.0 if angle is None else angle,
+                  spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT, emode=0.0 if emode is None else emode)
+
+
+def simulate_fields(tables, phantom, fields, particle=PHOTON, energy=10.0, emin=0.0, emax=0.0, emode=None,
+                    field=100.0, gantry=True, seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, schedule=None,
+                    out=None, fetch=True):
+    """fields: list of (x0, y0, angle_deg, nhist); one run into one tally
+    (dpm_simulate_fields). Returns (tally, sum2_or_steps, stats dict)."""
+    src = _source(particle, energy, emin, emax, field, 0.0, 0.0, 0.0 if gantry else None, emode)
+    arr = (Field * len(fields))(*[Field(x0=f[0], y0=f[1], angle_deg=f[2], nhist=int(f[3])) for f in fields])
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch)
+    if schedule is not None:
+        opt.chunk, opt.waves_per_cu = schedule["chunk"], schedule["waves_per_cu"]
+        opt.refill_min, opt.tally_mode = schedule["refill_min"], schedule["tally_mode"]
+    st = Stats()
+    if not fetch:
+        _check(lib().dpm_simulate_fields(tables.h, phantom.h, ctypes.byref(src), arr, len(fields), ctypes.byref(opt),
+                                         None, None, ctypes.byref(st)))
+        return None, None, st.as_dict()
+    tally, second = out if out is not None else (np.zeros(phantom.nbins), np.zeros(phantom.nbins))
+    _check(lib().dpm_simulate_fields(tables.h, phantom.h, ctypes.byref(src), arr, len(fields), ctypes.byref(opt),
+                                     _ptr(tally), _ptr(second), ctypes.byref(st)))
+    if phantom.shape is not None:
+        tally, second = tally.reshape(phantom.shape), second.reshape(phantom.shape)
+    return tally, second, st.as_dict()
+
+
 def tally_fetch(tables, phantom, out=None):
     """The last run's (tally, steps_or_sum2) from the device (dpm_tally_fetch)."""
     tally, second = out if out is not None else (np.zeros(phantom.nbins), np.zeros(phantom.nbins))
@@ -282,10 +325,9 @@ def tally_fetch(tables, phantom, out=None):
 
 
 def autotune(tables, phantom, nsample=2000000, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0,
-             x0=0.0, y0=0.0, precision=DPM_FP32):
+             x0=0.0, y0=0.0, precision=DPM_FP32, angle=None, emode=None):
     """Fastest launch schedule for this source on this phantom (dpm_autotune); a dict usable as schedule=."""
-    src = Source(kind=SRC_FIELD if field > 0 else SRC_PENCIL, particle=particle, energy=energy, emin=emin, emax=emax,
-                 x0=x0, y0=y0, field_x=field, field_y=field)
+    src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode)
     sch = Schedule()
     _check(lib().dpm_autotune(tables.h, phantom.h, ctypes.byref(src), int(nsample), precision, ctypes.byref(sch)))
     return sch.as_dict()

@@ -14,6 +14,14 @@ Material indices refer to the tables of generator configuration 3
   MeV; entry points drawn without replacement from the 2 mm grid over
   160x160 mm2; history counts lognormal(ln 1e5, sigma=1.0) clipped to
   [1e4, 1e6] (sigma is not given by the configuration; 1.0 is this choice).
+* config5_fields(total, n_angles, seed): configuration 5's gantry plan --
+  36 angles every 10 degrees around the phantom centre, history counts
+  uneven in [1e7, 5e7] (uniform draws rescaled to the total, 1e9), each a
+  10x10 cm2 photon field with a triangular (0.1, 2, 18) MeV spectrum
+  (simulate_fields(..., emin=0.1, emode=2.0, emax=18.0, field=100.0)).
+  The phantom is the heterogeneous recipe at 512x512x256, 1 mm, with the
+  ellipsoid semi-axes scaled by 2 ("built like configuration 2 with scaled
+  ellipsoids" read as the heterogeneous recipe with scaled ellipsoids).
 """
 import numpy as np
 
@@ -69,3 +77,21 @@ def beamlets(n=512, seed=4321, median=1e5, sigma=1.0, grid_mm=2.0, field_mm=160.
         nh = int(np.clip(rng.lognormal(np.log(median), sigma), 1e4, 1e6))
         out.append((float(x), float(y), particle, e, nh))
     return out
+
+
+def config5_fields(total=1e9, n_angles=36, seed=5678, lo=1e7, hi=5e7):
+    """[(x0, y0, angle_deg, nhist)] of the gantry plan; counts in [lo, hi]
+    summing to `total` (scaled down together when total < n_angles * lo)."""
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(lo, hi, n_angles)
+    if total >= n_angles * lo:
+        for _ in range(100):   # rescale to the total, keeping every count in [lo, hi]
+            w = np.clip(w * (total / w.sum()), lo, hi)
+            if abs(w.sum() - total) < 1:
+                break
+    else:
+        w = w * (total / w.sum())
+    n = np.maximum(1, np.floor(w)).astype(np.int64)
+    n[0] += int(total) - int(n.sum())
+    return [(0.0, 0.0, 10.0 * i, int(n[i])) for i in range(n_angles)]
+

@@ -131,15 +131,24 @@ int64_t dpm_phantom_num_bins(const dpm_phantom* p);
 /* ---------------------------------------------------------------------------
  * Sources and runs
  * ------------------------------------------------------------------------- */
-enum { DPM_SRC_PENCIL = 0, DPM_SRC_FIELD = 1 };
+enum { DPM_SRC_PENCIL = 0, DPM_SRC_FIELD = 1, DPM_SRC_GANTRY = 2 };
+enum { DPM_SPEC_FLAT = 0, DPM_SPEC_TRIANGULAR = 1 };
 
 typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
   int kind;                        /* DPM_SRC_* */
   int particle;                    /* DPM_ELECTRON or DPM_PHOTON */
   double energy;                   /* MeV (mono-energetic when emax <= emin) */
-  double emin, emax;               /* uniform spectrum [emin, emax] when emax > emin */
+  double emin, emax;               /* spectrum over [emin, emax] when emax > emin */
   double x0, y0;                   /* entry point / field centre [mm] */
-  double field_x, field_y;         /* field size [mm] (DPM_SRC_FIELD) */
+  double field_x, field_y;         /* field size [mm] (DPM_SRC_FIELD, DPM_SRC_GANTRY; 0 = pencil) */
+  /* DPM_SRC_GANTRY (voxel phantoms): beam direction (sin a, 0, cos a) through
+     the phantom centre, the field (or pencil) on the plane through the
+     centre perpendicular to it, with axes (cos a, 0, -sin a) and (0, 1, 0);
+     primaries start where their ray enters the phantom. a = 0 is the +z
+     field of DPM_SRC_FIELD. */
+  double angle_deg;
+  int spectrum;                    /* DPM_SPEC_FLAT, or DPM_SPEC_TRIANGULAR(emin, emode, emax) */
+  double emode;
 } dpm_source;
 
 typedef struct dpm_run_options {
@@ -191,6 +200,22 @@ typedef struct dpm_stats {
  * on this handle) and dpm_tally_fetch copies it out. */
 int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
                  const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
+
+/* Several fields of one source (treatment-plan style: gantry angles, uneven
+ * history counts) in one run into one tally, as dpm_simulate: field i runs
+ * fields[i].nhist histories with its own entry point and angle, history ids
+ * consecutive from opt->hist_begin in field order (so the result equals the
+ * sum of dpm_simulate calls with those hist_begin values). `src` gives kind,
+ * particle, energy or spectrum and field size; its x0, y0, angle_deg are
+ * ignored. No counterpart in the reference (one pencil beam per run). */
+typedef struct dpm_field {
+  double x0, y0;
+  double angle_deg;
+  uint64_t nhist;
+} dpm_field;
+
+int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, const dpm_field* fields, int nf,
+                        const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
 
 /* Copies the tally of the last dpm_simulate on this handle (and, if
  * `second` is not NULL, its steps / sum-of-squares array) into host arrays of

@@ -1030,6 +1030,57 @@ int oracle_load(const char* dir) {
   return 0;
 }
 
+// Source extensions of the voxel phantom (not in the reference, whose
+// primaries are a pencil at the slab's front face, SimDPMLike.cc:64-75).
+// cos/sin of a gantry angle in degrees, exact at multiples of 90.
+void gantry_cos_sin(double deg, double* c, double* s) {
+  const double q = deg / 90.0;
+  if (q == std::floor(q)) {
+    const int k = ((int)std::fmod(q, 4.0) + 4) % 4;
+    const double cs[4] = {1.0, 0.0, -1.0, 0.0}, sn[4] = {0.0, 1.0, 0.0, -1.0};
+    *c = cs[k];
+    *s = sn[k];
+    return;
+  }
+  const double a = deg * (3.14159265358979323846 / 180.0);
+  *c = std::cos(a);
+  *s = std::sin(a);
+}
+
+// inverse CDF of the triangular distribution (lo, mode, hi)
+double triangular(double u, double lo, double mode, double hi) {
+  const double fc = (mode - lo) / (hi - lo);
+  if (u < fc) return lo + std::sqrt(u * (hi - lo) * (mode - lo));
+  return hi - std::sqrt((1.0 - u) * (hi - lo) * (hi - mode));
+}
+
+// Field point (fu, fv) on the plane through the phantom centre perpendicular
+// to the beam direction d = (sa, 0, ca), with axes a = (ca, 0, -sa) and
+// b = (0, 1, 0): the start point is where the ray of direction d through it
+// enters the box [-h, h] (slab method), moved 1e-3 mm inside; false if the
+// ray misses the box.
+bool gantry_entry(double fu, double fv, double ca, double sa, const double h[3], double pos[3], double dir[3]) {
+  const double q[3] = {fu * ca, fv, -fu * sa};
+  const double d[3] = {sa, 0.0, ca};
+  double tin = -1e300, tout = 1e300;
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] == 0.0) {
+      if (q[i] <= -h[i] || q[i] >= h[i]) return false;
+      continue;
+    }
+    const double t1 = (-h[i] - q[i]) / d[i], t2 = (h[i] - q[i]) / d[i];
+    tin = std::max(tin, std::min(t1, t2));
+    tout = std::min(tout, std::max(t1, t2));
+  }
+  if (!(tin + 2e-3 < tout)) return false;
+  const double t = tin + 1e-3;
+  for (int i = 0; i < 3; ++i) {
+    pos[i] = q[i] + t * d[i];
+    dir[i] = d[i];
+  }
+  return true;
+}
+
 int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
   if (!g.loaded) { g_err = "tables not loaded"; return 1; }
   Sim s;
@@ -1066,8 +1117,15 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
     g_err = "mt19937 mode reproduces the reference pencil beam in the slab only";
     return 1;
   }
+  if (cfg->src_kind == ORACLE_SRC_GANTRY && geo.mode != ORACLE_GEOM_VOXEL) {
+    g_err = "gantry fields need a voxel phantom";
+    return 1;
+  }
   const double zstart = geo.mode == ORACLE_GEOM_SLAB ? -0.5 * cfg->lbox : geo.org[2];
+  double ca = 1.0, sa = 0.0;
+  gantry_cos_sin(cfg->angle_deg, &ca, &sa);
   int curBatch = 0;
+  double ep = 0.0;
   for (uint64_t ih = 0; ih < cfg->nhist; ++ih) {
     if (cfg->nbatch > 0 && geo.mode == ORACLE_GEOM_VOXEL) {
       const int b = (int)((ih * (uint64_t)cfg->nbatch) / cfg->nhist);
@@ -1089,12 +1147,24 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
     p.ekin = cfg->e0;
     if (cfg->rng_mode == ORACLE_RNG_PHILOX) {
       s.rng.cur = &p;
-      if (cfg->src_kind == ORACLE_SRC_FIELD) {
+      if (cfg->src_kind == ORACLE_SRC_FIELD || cfg->src_kind == ORACLE_SRC_GANTRY) {
         const double ux = s.rng.u(), uy = s.rng.u();
         p.pos[0] = cfg->x0 + (ux - 0.5) * cfg->fx;
         p.pos[1] = cfg->y0 + (uy - 0.5) * cfg->fy;
       }
-      if (cfg->emax > cfg->emin) p.ekin = cfg->emin + s.rng.u() * (cfg->emax - cfg->emin);
+      if (cfg->emax > cfg->emin)
+        p.ekin = cfg->spectrum == ORACLE_SPEC_TRIANGULAR ? triangular(s.rng.u(), cfg->emin, cfg->emode, cfg->emax)
+                                                         : cfg->emin + s.rng.u() * (cfg->emax - cfg->emin);
+      ep += p.ekin;
+      if (cfg->src_kind == ORACLE_SRC_GANTRY) {
+        const double h[3] = {-geo.org[0], -geo.org[1], -geo.org[2]};
+        if (!gantry_entry(p.pos[0], p.pos[1], ca, sa, h, p.pos, p.dir)) {
+          // the ray misses the phantom: the primary leaves without interacting
+          s.escaped += p.ekin;
+          if (out->hist_edep) out->hist_edep[ih] = 0.0;
+          continue;
+        }
+      }
     }
     while (!s.stack.empty()) {
       Track tr = s.stack.back();
@@ -1115,6 +1185,7 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
   out->hist_size = geo.histSize;
   out->max_iz = geo.maxIz;
   out->e_escaped = s.escaped;
+  out->e_primary = cfg->rng_mode == ORACLE_RNG_PHILOX ? ep : cfg->e0 * (double)cfg->nhist;
   out->e_scored = geo.scored;
   out->n_pair = s.nPair;
   out->n_annih = s.nAnnih;

@@ -26,7 +26,8 @@ extern "C" {
 
 enum { ORACLE_RNG_MT19937 = 0, ORACLE_RNG_PHILOX = 1 };
 enum { ORACLE_GEOM_SLAB = 0, ORACLE_GEOM_VOXEL = 1 };
-enum { ORACLE_SRC_PENCIL = 0, ORACLE_SRC_FIELD = 1 };
+enum { ORACLE_SRC_PENCIL = 0, ORACLE_SRC_FIELD = 1, ORACLE_SRC_GANTRY = 2 };
+enum { ORACLE_SPEC_FLAT = 0, ORACLE_SPEC_TRIANGULAR = 1 };
 
 typedef struct oracle_cfg {
   int      rng_mode;      /* ORACLE_RNG_* */
@@ -47,6 +48,12 @@ typedef struct oracle_cfg {
   uint64_t hist_begin;    /* first global history id (Philox) */
   uint64_t nhist;
   int      nbatch;        /* >0: per-batch sum / sum^2 of the voxel tally */
+  /* gantry field (voxel phantom): beam direction (sin a, 0, cos a) through the
+     phantom centre, field on the plane through the centre, primaries start
+     where their ray enters the phantom */
+  double   angle_deg;
+  int      spectrum;      /* ORACLE_SPEC_*: with emax > emin, flat or triangular(emin, emode, emax) */
+  double   emode;
 } oracle_cfg;
 
 typedef struct oracle_out {
@@ -64,6 +71,7 @@ typedef struct oracle_out {
   /* byte-model counters: voxel visits, deposits, SB, Moller, KN, GS samples,
      secondaries created, electron steps, photon steps */
   uint64_t cnt[9];
+  double   e_primary;     /* sum of the primaries' kinetic energies */
 } oracle_out;
 
 int  oracle_load(const char* dir);          /* 0 on success */

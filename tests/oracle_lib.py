@@ -14,7 +14,8 @@ CLI_PATH = os.path.join(ROOT, "oracle", "build", "dpm_oracle_cli")
 
 RNG_MT, RNG_PHILOX = 0, 1
 GEOM_SLAB, GEOM_VOXEL = 0, 1
-SRC_PENCIL, SRC_FIELD = 0, 1
+SRC_PENCIL, SRC_FIELD, SRC_GANTRY = 0, 1, 2
+SPEC_FLAT, SPEC_TRIANGULAR = 0, 1
 
 NCNT = 9
 COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries",
@@ -30,7 +31,8 @@ class Cfg(ctypes.Structure):
                 ("src_kind", ctypes.c_int), ("ptype", ctypes.c_int),
                 ("e0", ctypes.c_double), ("emin", ctypes.c_double), ("emax", ctypes.c_double),
                 ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double),
-                ("hist_begin", ctypes.c_uint64), ("nhist", ctypes.c_uint64), ("nbatch", ctypes.c_int)]
+                ("hist_begin", ctypes.c_uint64), ("nhist", ctypes.c_uint64), ("nbatch", ctypes.c_int),
+                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double)]
 
 
 class Out(ctypes.Structure):
@@ -40,7 +42,7 @@ class Out(ctypes.Structure):
                 ("hist_cap", ctypes.c_int), ("hist_size", ctypes.c_int), ("max_iz", ctypes.c_int),
                 ("e_escaped", ctypes.c_double), ("e_scored", ctypes.c_double),
                 ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64),
-                ("cnt", ctypes.c_uint64 * NCNT)]
+                ("cnt", ctypes.c_uint64 * NCNT), ("e_primary", ctypes.c_double)]
 
 
 _lib = None
@@ -107,14 +109,20 @@ def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_
 
 
 def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
-              seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0):
+              seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None):
+    """Voxel run in Philox mode. angle (degrees) selects the gantry source (a
+    field, or a pencil when field == 0, whose ray enters the phantom from the
+    side the beam direction (sin a, 0, cos a) points away from); emode selects
+    the triangular spectrum (emin, emode, emax)."""
     load(datadir)
     mat = np.ascontiguousarray(mat, dtype=np.uint8)
     rho = np.ascontiguousarray(rho, dtype=np.float32)
     nz, ny, nx = mat.shape
     cfg = Cfg(rng_mode=RNG_PHILOX, arg_rtl=0, seed=seed, geom_mode=GEOM_VOXEL, lbox=lbox,
               nx=nx, ny=ny, nz=nz, vox_mat=_p(mat), vox_rho=_p(rho), majorant_scale=majorant_scale,
-              src_kind=SRC_FIELD if field > 0 else SRC_PENCIL, ptype=ptype, e0=e0, emin=emin, emax=emax,
+              src_kind=SRC_GANTRY if angle is not None else (SRC_FIELD if field > 0 else SRC_PENCIL),
+              angle_deg=0.0 if angle is None else angle, spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT,
+              emode=0.0 if emode is None else emode, ptype=ptype, e0=e0, emin=emin, emax=emax,
               fx=field, fy=field, x0=x0, y0=y0, hist_begin=hist_begin, nhist=nhist, nbatch=nbatch)
     edep = np.zeros(mat.size)
     bs = np.zeros(mat.size) if nbatch > 0 else None
@@ -125,7 +133,7 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
         raise RuntimeError(lib().oracle_error().decode())
     return dict(edep=edep.reshape(mat.shape), batch_sum=bs, batch_sum2=bs2, hist_edep=hed,
                 e_escaped=out.e_escaped, e_scored=out.e_scored, n_pair=out.n_pair, n_annih=out.n_annih,
-                counters=dict(zip(COUNTER_NAMES, list(out.cnt))))
+                counters=dict(zip(COUNTER_NAMES, list(out.cnt))), e_primary=out.e_primary)
 
 
 def sample_many(kind, e, r, imat=0):

@@ -283,3 +283,73 @@ def test_config3_recipe_fp32_vs_fp64_gamma(gpu_tables3):
     assert pr >= 0.99, pr
     bal = _energy_balance(n, 12.0, st32["e_scored"], st32["e_escaped"], st32["n_pair"], st32["n_annih"])
     assert abs(bal) < 1e-5 * n * 12.0, bal
+
+
+@pytest.mark.gpu
+def test_gantry_field_fp64_history_by_history(gpu_tables3, tables3):
+    # 30-degree photon field, triangular (0.1, 2, 18) MeV spectrum, heterogeneous voxels
+    mat, rho = small_heterogeneous((20, 22, 24), seed=2)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 2.0)
+    scale = float(np.max(rho / np.array([1.0, 4.54, 1.85])[mat]))
+    n = 2000
+    kw = dict(emin=0.1, emax=18.0, field=20.0, seed=47)
+    tally, _, st, he = dpm_amd.simulate(gpu_tables3, ph, n, particle=0, angle=30.0, emode=2.0,
+                                        precision=dpm_amd.DPM_FP64, per_history=True, **kw)
+    ref = oracle_lib.run_voxel(tables3, mat, rho, 2.0, n, ptype=0, angle=30.0, emode=2.0, per_history=True,
+                               majorant_scale=scale, **kw)
+    rel = np.abs(he - ref["hist_edep"]) / np.maximum(np.abs(ref["hist_edep"]), 1e-300)
+    ok = (rel < 1e-9) | ((np.abs(he) < 1e-300) & (np.abs(ref["hist_edep"]) < 1e-300))
+    assert np.mean(ok) >= 0.99, np.mean(ok)
+    np.testing.assert_allclose(st["e_primary"], ref["e_primary"], rtol=1e-12)
+    bal = _energy_balance(1, st["e_primary"], st["e_scored"], st["e_escaped"], st["n_pair"], st["n_annih"])
+    assert abs(bal) < 1e-9 * st["e_primary"], bal
+
+
+@pytest.mark.gpu
+def test_fields_run_equals_the_sum_of_single_fields(gpu_tables0):
+    mat, rho = water_cube(24)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    fields = [(0.0, 0.0, 0.0, 1500), (5.0, -3.0, 100.0, 2500), (-4.0, 2.0, 250.0, 1000)]
+    kw = dict(particle=0, emin=0.1, emax=18.0, emode=2.0, field=30.0, seed=51, precision=dpm_amd.DPM_FP64)
+    d, _, st = dpm_amd.simulate_fields(gpu_tables0, ph, fields, **kw)
+    total, off = np.zeros_like(d), 0
+    for x0, y0, a, n in fields:
+        kw1 = dict(kw)
+        kw1.pop("emode")
+        di, _, _, _ = dpm_amd.simulate(gpu_tables0, ph, n, x0=x0, y0=y0, angle=a, emode=2.0, hist_begin=off, **kw1)
+        total += di
+        off += n
+    np.testing.assert_allclose(d, total, rtol=1e-9, atol=1e-12)
+    # batch statistics over a multi-field run
+    d8, s8, _ = dpm_amd.simulate_fields(gpu_tables0, ph, fields, nbatch=8, **kw)
+    np.testing.assert_allclose(d8, d, rtol=1e-9, atol=1e-12)
+    assert np.all(s8 + 1e-12 >= d8 * d8 / 8 * (1 - 1e-9))
+
+
+@pytest.mark.gpu
+def test_gantry_rays_missing_the_phantom(gpu_tables0):
+    mat, rho = water_cube(10)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    d, _, st, _ = dpm_amd.simulate(gpu_tables0, ph, 500, particle=0, energy=5.0, x0=50.0, angle=45.0, seed=1)
+    assert d.sum() == 0.0 and abs(st["e_escaped"] - 2500.0) < 1e-6
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.simulate(gpu_tables0, dpm_amd.Phantom.slab(gpu_tables0, 1.0, 0), 10, particle=0, energy=5.0,
+                         angle=10.0)
+
+
+@pytest.mark.gpu
+def test_config5_recipe_fp32_vs_fp64_gamma(gpu_tables3):
+    # configuration 5 at reduced size: heterogeneous phantom, 36 gantry angles
+    # with uneven history counts, triangular (0.1, 2, 18) MeV photon fields
+    import phantoms
+    mat, rho = phantoms.heterogeneous(80, 80, 64, 2.0, seed=99, axis_scale=0.6)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 2.0)
+    fields = phantoms.config5_fields(total=1.8e6, lo=3e4, hi=9e4)
+    kw = dict(particle=0, emin=0.1, emax=18.0, emode=2.0, field=100.0, seed=53, nbatch=10)
+    d32, _, st32 = dpm_amd.simulate_fields(gpu_tables3, ph, fields, **kw)
+    d64, _, _ = dpm_amd.simulate_fields(gpu_tables3, ph, fields, precision=dpm_amd.DPM_FP64, **kw)
+    pr = gamma.gamma_pass_rate_3d(d64 / rho, d32 / rho, spacing_mm=2.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    assert pr >= 0.99, pr
+    bal = _energy_balance(1, st32["e_primary"], st32["e_scored"], st32["e_escaped"], st32["n_pair"],
+                          st32["n_annih"])
+    assert abs(bal) < 1e-5 * st32["e_primary"], bal
