@@ -44,6 +44,43 @@ def algorithmic_bytes(c):
     return sum(BYTES[k] * c[k] for k in BYTES)
 
 
+def atomic_roofline(dpm_amd, t, st, footprint):
+    """FP64 tally atomics of the electron kernel per second of that kernel,
+    against the atomic probe's throughput for the same number of additions
+    spread over the tally's footprint (rows x voxels) and over an L2-resident
+    buffer (2 MiB): the chip's ceilings for this access pattern."""
+    n_at = st["counters_electron"]["tally_atomics"]
+    ms_el = st["kernel_ms"][2]
+    if n_at <= 0 or ms_el <= 0:
+        return None
+    achieved = n_at / (ms_el * 1e-3)
+    sec_fp, _ = dpm_amd.probe_atomics(t, min(footprint, (1 << 32) - 1), n_at)
+    sec_l2, _ = dpm_amd.probe_atomics(t, 1 << 18, n_at)
+    return {"unit": "FP64 atomic adds/s", "achieved": achieved, "tally_atomics_per_step": n_at,
+            "peak_same_footprint": n_at / sec_fp, "peak_l2_resident": n_at / sec_l2,
+            "frac": achieved / (n_at / sec_fp), "footprint_cells": int(footprint)}
+
+
+def stream_copy_gbs(gib=1.0, reps=3):
+    """Device-to-device copy rate (read + write bytes / s) of a gib-GiB buffer:
+    the measured HBM ceiling beside the 8 TB/s datasheet figure."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    best = 1e30
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * n * 4 / best / 1e9
+
+
 def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
     import oracle_lib  # the checker's CLI, timed as the CPU baseline
     cli = oracle_lib.CLI_PATH
@@ -159,6 +196,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}), flush=True)
         del src, dst
+    # torch measures the copy rate before the engine opens the device
+    copy_gbs = stream_copy_gbs()
     tables0 = __graft_entry__.tables_dir(0)
     nx, lbox, field, emin, emax = 150, 2.0, 100.0, 0.5, 20.0
     nhist = int(args.histories)
@@ -179,8 +218,9 @@ def main():
         sched = {"chunk": args.chunk, "waves_per_cu": args.waves_per_cu, "refill_min": args.refill_min,
                  "tally_mode": max(0, args.tally_mode), "source": "command line / defaults"}
     else:
-        sched = dpm_amd.autotune(t, ph, nsample=2000000, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field)
-        sched["source"] = "dpm_autotune on 2e6 histories of this workload"
+        ns = min(nhist, 1 << 24)
+        sched = dpm_amd.autotune(t, ph, nsample=ns, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field)
+        sched["source"] = "dpm_autotune on %d histories of this workload" % ns
         sched["chunk"] = 0
 
     def step(precision=dpm_amd.DPM_FP32, seed=1, fetch=True):
@@ -217,6 +257,7 @@ def main():
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    atomics = atomic_roofline(dpm_amd, t, st, ph.nbins * max(1, args.nbatch))
 
     result = {
         "metric": "histories/s", "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
@@ -228,7 +269,8 @@ def main():
                    "histories": nhist, "voxels": nx ** 3, "nbatch": args.nbatch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches},
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches,
+                     "measured_copy_GBs": copy_gbs, "atomics": atomics},
         "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
         "device_seconds_per_step": st["seconds"],
         "pcie_inclusive": {"value": args.steps * nhist / wall_pcie, "ms_per_step": wall_pcie / args.steps * 1e3,

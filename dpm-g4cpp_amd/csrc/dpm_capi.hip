@@ -363,6 +363,17 @@ __global__ void k_fold(double* rows, int nrows, int64_t nvox, double* sum, doubl
   }
 }
 
+// FP64 atomic-add throughput probe: nops device-scope additions to cells
+// spread pseudo-randomly over a buffer of ncells doubles (< 2^32), the access
+// pattern of the voxel tally, without return values as the transport issues them
+__global__ void k_atomic_probe(double* buf, uint64_t ncells, uint64_t nops, uint64_t seed) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nops; i += stride) {
+    const uint64_t h = mix64(i ^ seed);
+    unsafeAtomicAdd(&buf[((h >> 32) * ncells) >> 32], 1.0);
+  }
+}
+
 // sum of a tally into *out (block partial sums, one atomic per block)
 __global__ void k_sum(const double* x, int64_t n, double* out) {
   __shared__ double part[4];
@@ -729,7 +740,11 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.stats = t->stats;
   P.overflow = t->ctl + 5;
   P.refill_min = std::min(64, std::max(1, opt->refill_min));
-  P.tally_mode = opt->tally_mode == 1 ? 1 : 0;
+  P.tally_mode = (opt->tally_mode == 1 || opt->tally_mode == 2) ? opt->tally_mode : 0;
+  // FP32: the electron kernel's spline tables go to LDS when they fit
+  P.lds_tables = (sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO &&
+                  3 * T.nmat * T.itr1.n + T.moller.n + T.k1.n <= kLdsSplineNodes &&
+                  T.itr1.n == T.dedx.n && T.itr1.n == T.brem.n) ? 1 : 0;
 
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
@@ -885,6 +900,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
     st->kernel_calls[0] = 0; st->kernel_calls[1] = c_ph; st->kernel_calls[2] = c_el; st->kernel_calls[3] = c_fold;
     st->graphs = n_graphs;
+    st->lds_tables = P.lds_tables;
   }
   return DPM_OK;
 }
@@ -970,6 +986,43 @@ int dpm_host_alloc(size_t bytes, void** out) {
 
 int dpm_host_free(void* p) {
   if (p && hipHostFree(p) != hipSuccess) return set_error(DPM_E_DEVICE, "hipHostFree failed");
+  return DPM_OK;
+}
+
+int dpm_probe_atomics(dpm_tables* t, int64_t ncells, int64_t nops, double* seconds, double* check) {
+  if (!t || !seconds) return set_error(DPM_E_INVALID, "null argument");
+  if (ncells < 1 || ncells >= ((int64_t)1 << 32) || nops < 1 || nops > ((int64_t)1 << 40))
+    return set_error(DPM_E_INVALID, "probe needs 1 <= ncells < 2^32 and 1 <= nops <= 2^40");
+  HIP_TRY(hipSetDevice(t->device));
+  double* buf = nullptr;
+  if (hipMalloc(&buf, sizeof(double) * (size_t)ncells) != hipSuccess)
+    return set_error(DPM_E_NOMEM, "cannot allocate the probe buffer");
+  struct Free {
+    double* p;
+    ~Free() { (void)hipFree(p); }
+  } guard{buf};
+  HIP_TRY(hipMemsetAsync(buf, 0, sizeof(double) * (size_t)ncells, t->stream));
+  const int blocks = std::max(1, t->num_cu * 16);
+  // one untimed pass brings the buffer's pages and the code in
+  hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, t->stream, buf, (uint64_t)ncells,
+                     (uint64_t)std::min<int64_t>(nops, (int64_t)blocks * 256), (uint64_t)0x51);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(t->ev[0], t->stream));
+  hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, t->stream, buf, (uint64_t)ncells, (uint64_t)nops,
+                     (uint64_t)0x1234);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(t->ev[1], t->stream));
+  HIP_TRY(hipEventSynchronize(t->ev[1]));
+  *seconds = elapsed(t->ev[0], t->ev[1]) * 1e-3;
+  if (check) {
+    // total added: warm-up pass + timed pass (every addition is 1.0)
+    HIP_TRY(hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream));
+    hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, t->stream, buf, ncells,
+                       &t->stats->scored);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(check, &t->stats->scored, sizeof(double), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+  }
   return DPM_OK;
 }
 
@@ -1328,7 +1381,7 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
     return set_error(DPM_E_NOMEM, "cannot allocate the tally");
   static const int kWaves[] = {8, 12, 16};
   static const int kRefill[] = {1, 4, 16};
-  static const int kTally[] = {0, 1};
+  static const int kTally[] = {0, 1, 2};
   dpm_schedule best;
   std::memset(&best, 0, sizeof(best));
   double best_s = 1e300, default_s = 0.0;
@@ -1336,7 +1389,7 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
   for (int w : kWaves)
     for (int r : kRefill)
       for (int tm : kTally) {
-        if (tm == 1 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
+        if (tm != 0 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
         dpm_run_options opt;
         std::memset(&opt, 0, sizeof(opt));
         opt.seed = 0x5eed;

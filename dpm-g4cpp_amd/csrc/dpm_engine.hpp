@@ -35,6 +35,10 @@ struct QueueView {
   int cap;
 };
 
+// capacity of the LDS copy of the electron kernel's spline tables (nodes;
+// 20 B each): 3 * nmat * n + 2 * n = 1408 for 3 materials of 128 nodes
+constexpr int kLdsSplineNodes = 1536;
+
 template <typename R>
 struct KParams {
   Tables<R> T;
@@ -52,6 +56,7 @@ struct KParams {
   int kind;                 // 0 photon kernel, 1 electron kernel (counter slot)
   int refill_min;           // idle lanes that trigger a wave refill (1..64)
   int tally_mode;           // voxel tally: 0 per-lane write-combining, 1 one atomic per deposit
+  int lds_tables;           // FP32 electron kernel: spline tables copied into LDS (they fit)
 };
 
 // Primaries of one particle kind, enumerated in list order: entry i owns

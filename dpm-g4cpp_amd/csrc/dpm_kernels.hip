@@ -46,6 +46,9 @@ __device__ __noinline__ int2 pick_energy_bin_fp64(double e, double logMinE_d, do
   return make_int2(((double)k1 * (1.0 / 16777216.0) < fr) ? ie + 1 : ie, ie);
 }
 constexpr int kSlabLds = 1024;         // depth bins privatised in LDS
+struct NoLds {                         // placeholder of an LDS table a kernel variant does not use
+  int unused;
+};
 constexpr uint32_t kMaxIter = 200000;  // per-particle iteration cap (termination guard)
 
 // ---------------------------------------------------------------------------
@@ -179,9 +182,92 @@ struct WComb {
   double e;        // pending energy
 };
 
-template <int GEOM, class BlockAcc>
-__device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb& w, R e, const int box[3],
-                                         uint32_t tag, uint32_t hist) {
+// Per-wave LDS privatisation of hot tally cells (voxel tally mode 2): each
+// wave owns a direct-mapped table of kHotSlots (cell, FP64 sum) entries in
+// front of the global FP64 atomics. A write-combined sum goes to the slot of
+// its cell (row * nvox + voxel): it is added there when the slot holds that
+// cell; otherwise the lane takes the slot with a compare-and-swap on the key
+// and sends the evicted sum to HBM, or, when another lane of the wave took
+// the slot for a different cell in the same instruction, sends its own sum to
+// HBM. Only the owning wave touches its table, so wavefront-scope fences order
+// the steps; the wave flushes the table when it ends.
+constexpr int kHotSlots = 256;
+constexpr unsigned long long kHotEmpty = ~0ull;
+struct HotTable {
+  unsigned long long key[kBlock / 64][kHotSlots];
+  double val[kBlock / 64][kHotSlots];
+};
+template <int GEOM> struct HotOf { using type = HotTable; };
+template <> struct HotOf<GEOM_SLAB> { using type = NoLds; };
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+__device__ __forceinline__ void hot_init(HotTable& H, int wave) {
+  for (int i = lane_id(); i < kHotSlots; i += 64) {
+    H.key[wave][i] = kHotEmpty;
+    H.val[wave][i] = 0.0;
+  }
+  wave_fence();
+}
+
+template <class BlockAcc>
+__device__ __forceinline__ void hot_add(BlockAcc& a, const KParams<R>& P, HotTable& H, int wave, uint64_t k,
+                                        double e) {
+  unsigned long long* key = H.key[wave];
+  double* val = H.val[wave];
+  const int s = (int)((k * 0x9E3779B97F4A7C15ull) >> 56);   // 8 bits: kHotSlots = 256
+  const unsigned long long K = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  const bool hit = K == k;
+  if (hit) atomicAdd(&val[s], e);
+  wave_fence();
+  unsigned long long old = K;
+  if (!hit) old = atomicCAS(&key[s], K, (unsigned long long)k);
+  const bool won = !hit && old == K;
+  wave_fence();
+  double ov = 0.0;
+  if (won) ov = atomicExch(&val[s], e);
+  wave_fence();
+  const bool join = !hit && !won && old == k;
+  if (join) atomicAdd(&val[s], e);
+  if (won && K != kHotEmpty) {
+    count(a, kCntTallyAtomics);
+    unsafeAtomicAdd(&P.tally[K], ov);
+  }
+  if (!hit && !won && !join) {
+    count(a, kCntTallyAtomics);
+    unsafeAtomicAdd(&P.tally[k], e);
+  }
+}
+
+// end of a wave: every cached sum to HBM (the wave's lanes have all finished)
+template <class BlockAcc>
+__device__ __forceinline__ void hot_flush(BlockAcc& a, const KParams<R>& P, HotTable& H, int wave) {
+  wave_fence();
+  for (int i = lane_id(); i < kHotSlots; i += 64) {
+    const unsigned long long K = H.key[wave][i];
+    if (K != kHotEmpty) {
+      count(a, kCntTallyAtomics);
+      unsafeAtomicAdd(&P.tally[K], H.val[wave][i]);
+    }
+  }
+}
+
+// a write-combined sum to the tally: global atomic, or the wave's LDS table
+template <class BlockAcc, class Hot>
+__device__ __forceinline__ void tally_add(BlockAcc& a, const KParams<R>& P, Hot& H, int wave, int64_t k, double e) {
+  if constexpr (sizeof(Hot) == sizeof(HotTable)) {
+    if (P.tally_mode == 2) {
+      hot_add(a, P, H, wave, (uint64_t)k, e);
+      return;
+    }
+  }
+  count(a, kCntTallyAtomics);
+  unsafeAtomicAdd(&P.tally[k], e);
+}
+
+template <int GEOM, class BlockAcc, class Hot>
+__device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb& w, Hot& H, int wave, R e,
+                                         const int box[3], uint32_t tag, uint32_t hist) {
   if (GEOM == GEOM_SLAB || P.tally_mode == 1) {
     score<GEOM>(a, P, e, box, tag, hist);
     return;
@@ -189,10 +275,7 @@ __device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb
   count(a, kCntDeposits);
   const int64_t k = (int64_t)((size_t)tag * P.nvox + ((size_t)box[2] * P.G.ny + box[1]) * P.G.nx + box[0]);
   if (k != w.vox) {
-    if (w.vox >= 0) {
-      count(a, kCntTallyAtomics);
-      unsafeAtomicAdd(&P.tally[w.vox], w.e);
-    }
+    if (w.vox >= 0) tally_add(a, P, H, wave, w.vox, w.e);
     w.vox = k;
     w.e = 0.0;
   }
@@ -200,11 +283,10 @@ __device__ __forceinline__ void score_wc(BlockAcc& a, const KParams<R>& P, WComb
   if (P.hist_edep) unsafeAtomicAdd(&P.hist_edep[hist - P.hist_base], (double)e);
 }
 
-template <class BlockAcc>
-__device__ __forceinline__ void flush_wc(BlockAcc& a, const KParams<R>& P, WComb& w) {
+template <class BlockAcc, class Hot>
+__device__ __forceinline__ void flush_wc(BlockAcc& a, const KParams<R>& P, WComb& w, Hot& H, int wave) {
   if (w.vox >= 0) {
-    count(a, kCntTallyAtomics);
-    unsafeAtomicAdd(&P.tally[w.vox], w.e);
+    tally_add(a, P, H, wave, w.vox, w.e);
     w.vox = -1;
     w.e = 0.0;
   }
@@ -579,23 +661,95 @@ struct EState {
   bool hinge;
 };
 
-template <int GEOM, class BlockAcc>
-__device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, Track& t, EState& s) {
+// The five spline tables of the electron step (SimDataSpline, with the
+// clamps of SimITr1MFPElastic.hh:35-47, SimStoppingPower.hh:36-54,
+// SimIMFPBrem.hh:35-53, SimIMFPMoller.hh:35-47, SimMaxScatStrength.hh:39-51),
+// read either from the device table pack or from a copy in LDS. The LDS copy
+// (FP32 build) holds the node abscissae and (y, b, c, d) of ITr1, dE/dx and
+// brem IMFP (nmat materials each), then Moller IMFP and K1; each block loads
+// it once when the kernel starts, and a lookup is then one ds_read_b32 plus
+// one ds_read_b128 instead of two loads through L1/L2. The arithmetic is the
+// same in both forms, so they give identical results.
+struct LdsSplines {
+  float x[kLdsSplineNodes];
+  float4 q[kLdsSplineNodes];
+};
+template <bool LDS> struct LdsOf { using type = NoLds; };
+template <> struct LdsOf<true> { using type = LdsSplines; };
+
+// LDS node offsets of the five tables (order above)
+__device__ __forceinline__ int4 lds_spline_offsets(const Tables<R>& T) {
+  const int o_dedx = T.nmat * T.itr1.n;
+  const int o_brem = o_dedx + T.nmat * T.dedx.n;
+  const int o_mol = o_brem + T.nmat * T.brem.n;
+  return make_int4(o_dedx, o_brem, o_mol, o_mol + T.moller.n);
+}
+
+template <bool LDS>
+struct ESplines {
+  const Tables<R>& T;
+  const LdsSplines* L;
+  int4 off;   // dE/dx, brem, Moller, K1 (ITr1 at 0)
+
+  __device__ __forceinline__ R value(const SplineTab<R>& t, int base, R e, int imat) const {
+    if (!LDS) return spline_value(t, e, imat);
+    const R ee = rmin(t.emaxm, rmax(t.emin, e));
+    const R lx = rlog(ee);
+    int i = (int)((lx - t.logMinX) * t.invLogDelta);
+    i = max(0, min(t.n - 2, i));
+    const int j = base + imat * t.n + i;
+    const float x = L->x[j];
+    const float4 q = L->q[j];
+    const R dx = ee - (R)x;
+    return rmax((R)1.0E-20, (R)q.x + dx * ((R)q.y + dx * ((R)q.z + dx * (R)q.w)));
+  }
+  __device__ __forceinline__ R itr1(R e, int m) const { return value(T.itr1, 0, e, m); }
+  __device__ __forceinline__ R dedx(R e, int m) const { return value(T.dedx, off.x, e, m); }
+  __device__ __forceinline__ R brem(R e, int m) const { return value(T.brem, off.y, e, m); }
+  __device__ __forceinline__ R moller(R e) const { return value(T.moller, off.z, e, 0); }
+  __device__ __forceinline__ R k1(R e) const { return value(T.k1, off.w, e, 0); }
+  // Moller IMFP or K1 through one evaluation site
+  __device__ __forceinline__ R moller_or_k1(bool mol, R e) const {
+    if (!LDS) return spline_value(pick_spline(mol, T.moller, T.k1), e, 0);
+    return value(pick_spline(mol, T.moller, T.k1), mol ? off.z : off.w, e, 0);
+  }
+};
+
+// copy of the five spline tables into the block's LDS (all threads)
+__device__ __forceinline__ void load_lds_splines(const Tables<R>& T, LdsSplines& L) {
+  const int4 o = lds_spline_offsets(T);
+  const int total = o.w + T.k1.n;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const SplineNode<R>* nd;
+    if (i < o.x) nd = T.itr1.node + i;
+    else if (i < o.y) nd = T.dedx.node + (i - o.x);
+    else if (i < o.z) nd = T.brem.node + (i - o.y);
+    else if (i < o.w) nd = T.moller.node + (i - o.z);
+    else nd = T.k1.node + (i - o.w);
+    L.x[i] = (float)nd->x;
+    L.q[i] = make_float4((float)nd->y, (float)nd->b, (float)nd->c, (float)nd->d);
+  }
+}
+
+template <int GEOM, bool LDS, class BlockAcc>
+__device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, const ESplines<LDS>& SP, Track& t,
+                                              EState& s) {
   if (!start_track<GEOM>(a, P, t)) return false;
   s.hinge = true;
   s.E0 = t.E;
-  s.nTr1 = spline_value(P.T.k1, t.E, 0);
+  s.nTr1 = SP.k1(t.E);
   s.nTr10 = t.rng.uni<R>() * s.nTr1;
   s.nTr1 -= s.nTr10;
   s.nMol = -t.rng.lnu<R>();
-  s.invMol = spline_value(P.T.moller, t.E, 0);
+  s.invMol = SP.moller(t.E);
   s.nBrem = -t.rng.lnu<R>();
   return true;
 }
 
-template <int GEOM, class BlockAcc>
-__device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, Track& t, EState& s, WComb& wc,
-                                              const OutQ<QueueView<R>>& qe, const OutQ<QueueView<R>>& qg) {
+template <int GEOM, bool LDS, class BlockAcc, class Hot>
+__device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, const ESplines<LDS>& SP, Track& t,
+                                              EState& s, WComb& wc, Hot& H, int wave, const OutQ<QueueView<R>>& qe,
+                                              const OutQ<QueueView<R>>& qg) {
   const Tables<R>& T = P.T;
   // distance to boundary at the start of every step (SimDPMLike.cc:257, 451)
   const R stepGeom = distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
@@ -609,20 +763,20 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   const R rho = voxel_density<R, GEOM>(P.G, imat, t.box);
   R stepLength = stepGeom;
   int what = 0;
-  R theDEDX = spline_value(T.dedx, t.E, imat) * rho;
+  R theDEDX = SP.dedx(t.E, imat) * rho;
   R midStepE = rmax(t.E - (R)0.5 * stepLength * theDEDX, ecut);
-  R delNumTr1MFP = spline_value(T.itr1, midStepE, imat) * rho;
+  R delNumTr1MFP = SP.itr1(midStepE, imat) * rho;
   const R delNumMollerMFP = s.invMol * scalMolMFP * rho;
-  const R delNumBremMFP = spline_value(T.brem, midStepE, imat) * rho;
+  const R delNumBremMFP = SP.brem(midStepE, imat) * rho;
   const R stepBrem = s.nBrem / delNumBremMFP;
   if (stepBrem < stepLength) { stepLength = stepBrem; what = 1; }
   const R stepMoller = s.nMol / delNumMollerMFP;
   if (stepMoller < stepLength) { stepLength = stepMoller; what = 2; }
   R stepElastic = s.nTr1 / delNumTr1MFP;
   if (stepElastic < stepLength) {
-    stepElastic = s.nTr1 / (spline_value(T.itr1, t.E, imat) * rho);
+    stepElastic = s.nTr1 / (SP.itr1(t.E, imat) * rho);
     midStepE = rmax(t.E - (R)0.5 * stepElastic * theDEDX, ecut);
-    delNumTr1MFP = spline_value(T.itr1, midStepE, imat) * rho;
+    delNumTr1MFP = SP.itr1(midStepE, imat) * rho;
     stepElastic = rmin(stepLength, s.nTr1 / delNumTr1MFP);
     stepLength = stepElastic;
     what = 3;
@@ -631,7 +785,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   s.nMol = (what == 2) ? (R)0 : s.nMol - stepLength * delNumMollerMFP;
   s.nTr1 = (what == 3) ? (R)0 : s.nTr1 - stepLength * delNumTr1MFP;
   midStepE = rmax(t.E - (R)0.5 * stepLength * theDEDX, ecut);
-  theDEDX = spline_value(T.dedx, midStepE, imat) * rho;
+  theDEDX = SP.dedx(midStepE, imat) * rho;
   const R deltE = stepLength * theDEDX;
   const R eFinal = t.E - deltE;
   R edep;
@@ -657,7 +811,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   t.pos[0] += t.dir[0] * stepLength;
   t.pos[1] += t.dir[1] * stepLength;
   t.pos[2] += t.dir[2] * stepLength;
-  score_wc<GEOM>(a, P, wc, edep, t.box, t.tag, t.rng.hist);
+  score_wc<GEOM>(a, P, wc, H, wave, edep, t.box, t.tag, t.rng.hist);
   if (what == 0) return true;
   if (what == 4) return false;
   // The interaction switch of Simulate (SimDPMLike.cc:169-244). PerformBrem
@@ -738,7 +892,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   }
   if (what == 1 && t.E < ecut) {
     // below the cut after the brem: deposit the rest (:179-188)
-    score_wc<GEOM>(a, P, wc, t.E, t.box, t.tag, t.rng.hist);
+    score_wc<GEOM>(a, P, wc, H, wave, t.E, t.box, t.tag, t.rng.hist);
     t.E = (R)0.0;
     if (t.type == 2) annihilate(a, P, t, qg);
     return false;
@@ -751,7 +905,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   // Moller IMFP after a Moller step, or K1 at the end of an MSC step: one
   // spline evaluation at the current energy (both tables are material-free)
   if (what == 2 || (what == 3 && !hinge)) {
-    const R v = spline_value(pick_spline(what == 2, T.moller, T.k1), t.E, 0);
+    const R v = SP.moller_or_k1(what == 2, t.E);
     if (what == 2) {
       s.invMol = v;
     } else {
@@ -765,13 +919,23 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   return true;
 }
 
-template <int GEOM>
+template <int GEOM, bool LDS>
 __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES
 k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head,
             QueueView<R> qe, QueueView<R> qg) {
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
+  __shared__ typename LdsOf<LDS>::type lsp;
+  if constexpr (LDS) load_lds_splines(P.T, lsp);   // made visible by acc_init's barrier
+  ESplines<LDS> SP{P.T, nullptr, make_int4(0, 0, 0, 0)};
+  if constexpr (LDS) {
+    SP.L = &lsp;
+    SP.off = lds_spline_offsets(P.T);
+  }
   const int wave = threadIdx.x >> 6;
+  __shared__ typename HotOf<GEOM>::type hot;
+  if constexpr (GEOM == GEOM_VOXEL)
+    if (P.tally_mode == 2) hot_init(hot, wave);
   if (lane_id() == 0) {
     wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
     wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
@@ -796,7 +960,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
           return false;
         }
         t.rng.topup();
-        return electron_init<GEOM>(acc, P, t, s);
+        return electron_init<GEOM, LDS>(acc, P, SP, t, s);
       });
     if (!__any(has)) {
       if (exhausted) break;
@@ -804,11 +968,13 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     }
     if (has) {
       t.rng.topup();
-      has = electron_step<GEOM>(acc, P, t, s, wc, oe, og);
+      has = electron_step<GEOM, LDS>(acc, P, SP, t, s, wc, hot, wave, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
-      if (!has) flush_wc(acc, P, wc);
+      if (!has) flush_wc(acc, P, wc, hot, wave);
     }
   }
+  if constexpr (GEOM == GEOM_VOXEL)
+    if (P.tally_mode == 2) hot_flush(acc, P, hot, wave);
   flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
@@ -935,10 +1101,23 @@ hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, const SourcePar
 hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, const SourceParams* S, const PrimRange& pr,
                                               int geom, int blocks, const QueueView<R>& in, int* head,
                                               const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st) {
+#if DPM_PREC == 32
+  if (P.lds_tables) {
+    if (geom == GEOM_SLAB)
+      hipLaunchKernelGGL((k_electrons<GEOM_SLAB, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
+                         qg);
+    else
+      hipLaunchKernelGGL((k_electrons<GEOM_VOXEL, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
+                         qg);
+    return hipGetLastError();
+  }
+#endif
   if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL(k_electrons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_electrons<GEOM_SLAB, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
+                       qg);
   else
-    hipLaunchKernelGGL(k_electrons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_electrons<GEOM_VOXEL, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
+                       qg);
   return hipGetLastError();
 }
 

@@ -26,7 +26,7 @@ EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_t
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
             "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_simulate_fields", "dpm_last_error",
-            "dpm_abi_version")
+            "dpm_abi_version", "dpm_probe_atomics")
 
 
 class DpmError(RuntimeError):
@@ -49,7 +49,7 @@ class RunOptions(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("hist_begin", ctypes.c_uint64), ("precision", ctypes.c_int),
                 ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
                 ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int),
-                ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int), ("table_memory", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -57,7 +57,7 @@ class Stats(ctypes.Structure):
                 ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 10),
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
-                ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int)]
+                ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int), ("lds_tables", ctypes.c_int)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
                     counters_photon=dict(zip(COUNTER_NAMES, list(self.counters_photon))),
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
-                    kernel_calls=list(self.kernel_calls), graphs=self.graphs)
+                    kernel_calls=list(self.kernel_calls), graphs=self.graphs, lds_tables=self.lds_tables)
 
 
 class Schedule(ctypes.Structure):
@@ -119,6 +119,7 @@ def lib():
         L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
                                             ctypes.POINTER(Schedule)]
         L.dpm_host_free.argtypes = [vp]
+        L.dpm_probe_atomics.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp]
         _lib = L
     return _lib
 
@@ -234,6 +235,15 @@ class _Pinned:
             self.ptr = None
 
 
+def probe_atomics(tables, ncells, nops):
+    """(seconds, buffer sum) of nops FP64 atomic additions spread over ncells
+    cells (dpm_probe_atomics); nops / seconds is the atomic throughput."""
+    sec = ctypes.c_double(0.0)
+    chk = ctypes.c_double(0.0)
+    _check(lib().dpm_probe_atomics(tables.h, int(ncells), int(nops), ctypes.byref(sec), ctypes.byref(chk)))
+    return sec.value, chk.value
+
+
 def pinned_empty(n, dtype=np.float64):
     """A 1-D numpy array of n elements in page-locked host memory (fast tally copies)."""
     dtype = np.dtype(dtype)
@@ -246,7 +256,7 @@ def pinned_empty(n, dtype=np.float64):
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
              out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
-             angle=None, emode=None):
+             angle=None, emode=None, table_memory=0):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -262,7 +272,7 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
                                                       schedule["refill_min"], schedule["tally_mode"])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
                      waves_per_cu=waves_per_cu, hist_edep=_ptr(he), refill_min=refill_min, tally_mode=tally_mode,
-                     graph=int(bool(graph)), rounds_per_sync=rounds_per_sync)
+                     graph=int(bool(graph)), rounds_per_sync=rounds_per_sync, table_memory=table_memory)
     if out is not None:
         tally, steps = out
         for a in (tally, steps):
@@ -293,12 +303,12 @@ def _source(particle, energy, emin, emax, field, x0, y0, angle, emode):
 
 def simulate_fields(tables, phantom, fields, particle=PHOTON, energy=10.0, emin=0.0, emax=0.0, emode=None,
                     field=100.0, gantry=True, seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, schedule=None,
-                    out=None, fetch=True):
+                    out=None, fetch=True, table_memory=0):
     """fields: list of (x0, y0, angle_deg, nhist); one run into one tally
     (dpm_simulate_fields). Returns (tally, sum2_or_steps, stats dict)."""
     src = _source(particle, energy, emin, emax, field, 0.0, 0.0, 0.0 if gantry else None, emode)
     arr = (Field * len(fields))(*[Field(x0=f[0], y0=f[1], angle_deg=f[2], nhist=int(f[3])) for f in fields])
-    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch)
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, table_memory=table_memory)
     if schedule is not None:
         opt.chunk, opt.waves_per_cu = schedule["chunk"], schedule["waves_per_cu"]
         opt.refill_min, opt.tally_mode = schedule["refill_min"], schedule["tally_mode"]
@@ -344,7 +354,8 @@ def autotune_beamlets(tables, phantom, beamlets, nsample=4000000, precision=DPM_
 
 
 def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
-                      waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0, graph=True):
+                      waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0, graph=True,
+                      table_memory=0):
     """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
     arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
                                        for b in beamlets])
@@ -352,7 +363,8 @@ def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, pre
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
                                                       schedule["refill_min"], schedule["tally_mode"])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
-                     refill_min=refill_min, tally_mode=tally_mode, graph=int(bool(graph)))
+                     refill_min=refill_min, tally_mode=tally_mode, graph=int(bool(graph)),
+                     table_memory=table_memory)
     row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
     st = Stats()
     _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,

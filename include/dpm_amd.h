@@ -160,12 +160,19 @@ typedef struct dpm_run_options {
   int waves_per_cu;                /* persistent waves per CU; 0 = automatic */
   double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
   int refill_min;                  /* idle lanes of a wave that trigger its refill (1..64); 0 = 1 */
-  int tally_mode;                  /* voxel tally: 0 per-lane write-combining, 1 atomic per deposit */
+  int tally_mode;                  /* voxel tally: 0 per-lane write-combining, 1 atomic per deposit,
+                                      2 write-combining then a per-wave LDS table of hot cells */
   int graph;                       /* 1: replay the transport rounds from captured hipGraphs (no
                                       per-kernel timings in dpm_stats) */
   int rounds_per_sync;             /* transport rounds enqueued (or captured) per host synchronisation:
                                       even, 2..8; 0 = 8 */
+  int table_memory;                /* DPM_TABLES_AUTO (0): FP32 runs copy the electron spline tables
+                                      into each block's LDS when they fit (3*nmat+2 tables of 128
+                                      nodes: up to 3 materials); DPM_TABLES_GLOBAL (1): read them from
+                                      device memory. Both give identical results. */
 } dpm_run_options;
+
+enum { DPM_TABLES_AUTO = 0, DPM_TABLES_GLOBAL = 1 };
 
 enum {
   DPM_CNT_VOXEL_VISITS = 0, DPM_CNT_DEPOSITS, DPM_CNT_SB, DPM_CNT_MOLLER, DPM_CNT_KN, DPM_CNT_GS,
@@ -186,6 +193,7 @@ typedef struct dpm_stats {
   uint64_t counters_photon[DPM_NCNT];    /* counters of the photon kernel */
   uint64_t counters_electron[DPM_NCNT];  /* counters of the electron kernel */
   int graphs;                      /* hipGraphs captured by the run (graph = 1) */
+  int lds_tables;                  /* 1: the electron kernel read its spline tables from LDS */
 } dpm_stats;
 
 /* Replaces Simulate(nprimary, eprimary, iselectron, lbox, mat, el, ph,
@@ -305,6 +313,15 @@ int dpm_eval(dpm_tables* t, int which, int precision, int n, const double* energ
  * runtime. Any host pointer is accepted by the entry points above. */
 int dpm_host_alloc(size_t bytes, void** out);
 int dpm_host_free(void* p);
+
+/* Measurement helper, no counterpart in the reference: the device time of
+ * `nops` FP64 atomic additions (no return value, device scope) to cells spread
+ * pseudo-randomly over a fresh buffer of `ncells` doubles (< 2^32), i.e. the
+ * access pattern of the voxel tally; nops / *seconds is the atomic throughput
+ * the tally runs against. With `check` not NULL, *check receives the sum of
+ * the buffer (every addition adds 1.0, including those of an untimed warm-up
+ * pass of min(nops, 4096 * CUs) additions). */
+int dpm_probe_atomics(dpm_tables* t, int64_t ncells, int64_t nops, double* seconds, double* check);
 
 const char* dpm_last_error(void);
 int dpm_abi_version(void);

@@ -174,6 +174,7 @@ def test_schedules_do_not_change_results(gpu_tables0):
     d0, _, _, h0 = dpm_amd.simulate(gpu_tables0, ph, 4000, **kw)
     for sched in (dict(chunk=1000, waves_per_cu=8, refill_min=16, tally_mode=1),
                   dict(chunk=0, waves_per_cu=12, refill_min=4, tally_mode=0),
+                  dict(chunk=0, waves_per_cu=12, refill_min=4, tally_mode=2),
                   dict(chunk=0, waves_per_cu=16, refill_min=64, tally_mode=1)):
         d, _, _, h = dpm_amd.simulate(gpu_tables0, ph, 4000, schedule=sched, **kw)
         np.testing.assert_allclose(h, h0, rtol=1e-9, atol=1e-9)
@@ -185,12 +186,12 @@ def test_autotune_returns_a_timed_candidate(gpu_tables0):
     mat, rho = water_cube(30)
     ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
     s = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0)
-    assert s["waves_per_cu"] in (8, 12, 16) and s["refill_min"] in (1, 4, 16) and s["tally_mode"] in (0, 1)
-    assert s["candidates"] == 18 and s["chunk"] > 0
+    assert s["waves_per_cu"] in (8, 12, 16) and s["refill_min"] in (1, 4, 16) and s["tally_mode"] in (0, 1, 2)
+    assert s["candidates"] == 27 and s["chunk"] > 0
     assert s["histories_per_s"] >= s["default_histories_per_s"] > 0
     bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000)]
     sb = dpm_amd.autotune_beamlets(gpu_tables0, ph, bl, nsample=5000)
-    assert sb["candidates"] == 18 and sb["histories_per_s"] > 0
+    assert sb["candidates"] == 27 and sb["histories_per_s"] > 0
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.autotune(gpu_tables0, ph, nsample=10, particle=0, emin=0.5, emax=20.0)
 
@@ -353,3 +354,65 @@ def test_config5_recipe_fp32_vs_fp64_gamma(gpu_tables3):
     bal = _energy_balance(1, st32["e_primary"], st32["e_scored"], st32["e_escaped"], st32["n_pair"],
                           st32["n_annih"])
     assert abs(bal) < 1e-5 * st32["e_primary"], bal
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hetero", [False, True])
+def test_lds_tables_match_device_tables(gpu_tables0, gpu_tables3, hetero):
+    # FP32 electron splines read from the block's LDS copy or from the device
+    # pack: the same arithmetic on the same values, so every history scores
+    # the same energy (up to the order of its FP64 tally additions)
+    if hetero:
+        t = gpu_tables3
+        mat, rho = small_heterogeneous()
+    else:
+        t = gpu_tables0
+        mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(t, mat, rho, 2.0)
+    kw = dict(particle=-1, energy=12.0, field=20.0, seed=23, per_history=True, nbatch=4)
+    d_lds, s_lds, st_lds, h_lds = dpm_amd.simulate(t, ph, 3000, table_memory=0, **kw)
+    d_dev, s_dev, st_dev, h_dev = dpm_amd.simulate(t, ph, 3000, table_memory=1, **kw)
+    assert st_lds["lds_tables"] == 1 and st_dev["lds_tables"] == 0
+    np.testing.assert_allclose(h_lds, h_dev, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(d_lds, d_dev, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(s_lds, s_dev, rtol=1e-12, atol=1e-12)
+    assert st_lds["counters"] == st_dev["counters"]
+    # the FP64 path never uses the LDS copy
+    _, _, st64, _ = dpm_amd.simulate(t, ph, 200, precision=dpm_amd.DPM_FP64, **kw)
+    assert st64["lds_tables"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pencil", [True, False])
+def test_hot_voxel_lds_tally_equals_global_atomics(gpu_tables0, pencil):
+    # tally mode 2 (per-wave LDS table of hot cells) only regroups the FP64
+    # additions of mode 0; a pencil beam keeps the table hot (hits, evictions,
+    # lanes of one wave racing for one slot), a broad field mostly misses it
+    mat, rho = water_cube(40)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    kw = dict(particle=-1, energy=15.0, field=0.0 if pencil else 60.0, seed=29, nbatch=5, per_history=True)
+    n = 20000
+    d0, q0, st0, h0 = dpm_amd.simulate(gpu_tables0, ph, n, tally_mode=0, **kw)
+    d2, q2, st2, h2 = dpm_amd.simulate(gpu_tables0, ph, n, tally_mode=2, **kw)
+    np.testing.assert_allclose(h2, h0, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(d2, d0, rtol=1e-11, atol=1e-12 * d0.max())
+    np.testing.assert_allclose(q2, q0, rtol=1e-11, atol=1e-12 * q0.max())
+    assert abs(st2["e_scored"] - st0["e_scored"]) < 1e-10 * st0["e_scored"]
+    c0, c2 = st0["counters_electron"], st2["counters_electron"]
+    assert c2["deposits"] == c0["deposits"]
+    # the table absorbs part of a pencil beam's write-combined sums
+    print("tally atomics, mode 2 / mode 0: %.3f" % (c2["tally_atomics"] / c0["tally_atomics"]), flush=True)
+    if pencil:
+        assert c2["tally_atomics"] < c0["tally_atomics"], (c2["tally_atomics"], c0["tally_atomics"])
+
+
+@pytest.mark.gpu
+def test_atomic_probe_counts_every_addition(gpu_tables0):
+    # the measurement helper behind the bench's atomic-throughput figure: every
+    # one of its additions lands (timed pass + warm-up pass of <= 4096 per CU)
+    ncells, nops = 1 << 20, 1 << 24
+    sec, total = dpm_amd.probe_atomics(gpu_tables0, ncells, nops)
+    assert sec > 0
+    assert nops < total <= nops + 4096 * 1024 and total == int(total)
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.probe_atomics(gpu_tables0, 1 << 32, 10)
