@@ -232,6 +232,7 @@ int build_pack(const dpm_table_desc* d, Tables<R>& T, void** dev) {
   T.ecut = (R)d->electron_cut;
   T.gcut = (R)d->gamma_cut;
   T.nmat = d->nmat;
+  T.lin_shared_grid = (d->gmax.n == d->ptotal[0].n && same_grid(d->gmax.x, d->ptotal[0].x, d->gmax.n)) ? 1 : 0;
   // upload and patch pointers
   unsigned char* base = nullptr;
   HIP_TRY(hipMalloc(&base, pb.blob.bytes.size()));
@@ -745,6 +746,8 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.lds_tables = (sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO &&
                   3 * T.nmat * T.itr1.n + T.moller.n + T.k1.n <= kLdsSplineNodes &&
                   T.itr1.n == T.dedx.n && T.itr1.n == T.brem.n) ? 1 : 0;
+  P.lds_photon = (sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO && T.lin_shared_grid &&
+                  (2 + 3 * T.nmat) * T.ptot.n <= kLdsLinearFloats) ? 1 : 0;
 
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
@@ -900,7 +903,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
     st->kernel_calls[0] = 0; st->kernel_calls[1] = c_ph; st->kernel_calls[2] = c_el; st->kernel_calls[3] = c_fold;
     st->graphs = n_graphs;
-    st->lds_tables = P.lds_tables;
+    st->lds_tables = P.lds_tables | (P.lds_photon << 1);
   }
   return DPM_OK;
 }
@@ -917,6 +920,8 @@ int finish_stats(dpm_tables* t, dpm_stats* st) {
     }
     st->n_pair = h.cnt[0][kCntPair] + h.cnt[1][kCntPair];
     st->n_annih = h.cnt[0][kCntAnnih] + h.cnt[1][kCntAnnih];
+    st->lane_slots[0] = h.cnt[0][kCntLaneSlots];
+    st->lane_slots[1] = h.cnt[1][kCntLaneSlots];
     st->e_escaped = h.escaped;
     st->e_primary = h.e_primary;
     st->e_scored = h.scored;

@@ -102,6 +102,7 @@ struct Tables {
   R mscale[kMaxMat];
   R ecut, gcut;
   int nmat;
+  int lin_shared_grid;          // the maximum and per-material photon IMFPs share one energy grid
 };
 
 // ---------------------------------------------------------------------------

@@ -13,7 +13,7 @@ namespace dpm {
 // atomics actually issued after write-combining)
 enum {
   kCntVoxelVisits = 0, kCntDeposits, kCntSB, kCntMoller, kCntKN, kCntGS, kCntSecondaries, kCntESteps, kCntGSteps,
-  kCntTallyAtomics, kCntPair, kCntAnnih, kNumCounters
+  kCntTallyAtomics, kCntPair, kCntAnnih, kCntLaneSlots, kNumCounters
 };
 
 // counters per kernel kind: 0 photons, 1 electrons
@@ -38,6 +38,11 @@ struct QueueView {
 // capacity of the LDS copy of the electron kernel's spline tables (nodes;
 // 20 B each): 3 * nmat * n + 2 * n = 1408 for 3 materials of 128 nodes
 constexpr int kLdsSplineNodes = 1536;
+// capacity of the LDS copy of the photon kernel's linear tables (floats): the
+// shared energy grid, the maximum IMFP and 3 IMFPs per material, n nodes each:
+// (2 + 3 * nmat) * n = 5120 for one material of 1024 nodes, 8192 for two;
+// 32 KB, so that four blocks of the photon kernel fit in a CU's 160 KB
+constexpr int kLdsLinearFloats = 8192;
 
 template <typename R>
 struct KParams {
@@ -57,6 +62,7 @@ struct KParams {
   int refill_min;           // idle lanes that trigger a wave refill (1..64)
   int tally_mode;           // voxel tally: 0 per-lane write-combining, 1 one atomic per deposit
   int lds_tables;           // FP32 electron kernel: spline tables copied into LDS (they fit)
+  int lds_photon;           // FP32 photon kernel: linear tables copied into LDS (they fit)
 };
 
 // Primaries of one particle kind, enumerated in list order: entry i owns

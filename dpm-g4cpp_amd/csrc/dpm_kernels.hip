@@ -471,18 +471,69 @@ __device__ __forceinline__ void flush_eprim(BlockAcc& a, double eprim) {
   if (lane_id() == 0 && eprim != 0.0) atomicAdd(&a.eprim, eprim);
 }
 
+// The photon kernel's linear tables (SimDataLinear::GetValue, SimDataLinear.hh:
+// 38-45, behind SimIMFPMaxPhoton.hh:36-40 and SimIMFPPhoton.hh:40-46), read
+// from the device pack or from a copy in LDS (FP32 build, when the maximum and
+// the per-material IMFPs share one energy grid): the grid, then the maximum
+// IMFP, then the total, Compton and pair IMFPs of every material. Same
+// arithmetic in both forms.
+struct LdsLinear {
+  float v[kLdsLinearFloats];
+};
+template <bool LDS> struct LdsLinOf { using type = NoLds; };
+template <> struct LdsLinOf<true> { using type = LdsLinear; };
+
+template <bool LDS>
+struct PTables {
+  const Tables<R>& T;
+  const float* L;
+
+  __device__ __forceinline__ R value(const LinTab<R>& t, int ybase, R e, int imat) const {
+    if (!LDS) return linear_value(t, e, imat);
+    const R ee = rmin(t.emaxm, rmax(t.emin, e));
+    int i = (int)((ee - t.minX) * t.invDelta);
+    i = max(0, min(t.n - 2, i));
+    const float* y = L + ybase + imat * t.n;
+    const R x0 = (R)L[i], x1 = (R)L[i + 1], y0 = (R)y[i], y1 = (R)y[i + 1];
+    return rmax((R)1.0E-20, (y1 - y0) * (ee - x0) / (x1 - x0) + y0);
+  }
+  __device__ __forceinline__ R gmax(R e) const { return value(T.gmax, T.gmax.n, e, 0); }
+  __device__ __forceinline__ R total(R e, int m) const { return value(T.ptot, 2 * T.gmax.n, e, m); }
+  __device__ __forceinline__ R compton(R e, int m) const {
+    return value(T.pcomp, 2 * T.gmax.n + T.nmat * T.ptot.n, e, m);
+  }
+  __device__ __forceinline__ R pair(R e, int m) const {
+    return value(T.ppair, 2 * T.gmax.n + 2 * T.nmat * T.ptot.n, e, m);
+  }
+};
+
+// copy of the photon tables into the block's LDS (all threads)
+__device__ __forceinline__ void load_lds_linear(const Tables<R>& T, LdsLinear& L) {
+  const int n = T.gmax.n, nm = T.nmat * n;
+  const int total = 2 * n + 3 * nm;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    float v;
+    if (i < n) v = (float)T.gmax.xy[2 * i];
+    else if (i < 2 * n) v = (float)T.gmax.xy[2 * (i - n) + 1];
+    else if (i < 2 * n + nm) v = (float)T.ptot.xy[2 * (i - 2 * n) + 1];
+    else if (i < 2 * n + 2 * nm) v = (float)T.pcomp.xy[2 * (i - 2 * n - nm) + 1];
+    else v = (float)T.ppair.xy[2 * (i - 2 * n - 2 * nm) + 1];
+    L.v[i] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // photons: KeepTrackingPhoton (SimDPMLike.cc:463-635), one Woodcock step per
 // loop trip; e-/e+ go to qe (this round's electron queue), annihilation
 // photons to qg (next round's photon queue)
 // ---------------------------------------------------------------------------
-template <int GEOM, class BlockAcc>
-__device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Track& t, const OutQ<QueueView<R>>& qe,
-                                            const OutQ<QueueView<R>>& qg) {
+template <int GEOM, bool LDS, class BlockAcc>
+__device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, const PTables<LDS>& PT, Track& t,
+                                            const OutQ<QueueView<R>>& qe, const OutQ<QueueView<R>>& qg) {
   const Tables<R>& T = P.T;
   const R kInvEMC2 = (R)(1.0 / kEMC2);
   count(a, kCntGSteps);
-  const R globalMaxMFP = (R)1.0 / (linear_value(T.gmax, t.E, 0) * P.majScale);
+  const R globalMaxMFP = (R)1.0 / (PT.gmax(t.E) * P.majScale);
   const R stepLength = -globalMaxMFP * t.rng.lnu<R>();
   t.pos[0] += t.dir[0] * stepLength;
   t.pos[1] += t.dir[1] * stepLength;
@@ -492,11 +543,11 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Tr
   const int imat = material_index<R, GEOM>(P.G, t.box);
   if (imat < 0) { escape(a, t.E); return false; }
   const R rho = voxel_density<R, GEOM>(P.G, imat, t.box);
-  const R totalIMFP = linear_value(T.ptot, t.E, imat) * rho;
+  const R totalIMFP = PT.total(t.E, imat) * rho;
   const R r1 = t.rng.uni<R>();
   R theProb = (R)1.0 - totalIMFP * globalMaxMFP;
   if (r1 < theProb) return true;
-  theProb += linear_value(T.pcomp, t.E, imat) * rho * globalMaxMFP;
+  theProb += PT.compton(t.E, imat) * rho * globalMaxMFP;
   if (r1 < theProb) {
     // Compton (SimDPMLike.cc:512-570)
     count(a, kCntKN);
@@ -534,7 +585,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, Tr
     t.dir[0] = u1; t.dir[1] = u2; t.dir[2] = u3;
     return true;
   }
-  theProb += linear_value(T.ppair, t.E, imat) * rho * globalMaxMFP;
+  theProb += PT.pair(t.E, imat) * rho * globalMaxMFP;
   if (r1 < theProb) {
     // pair production (SimDPMLike.cc:573-627)
     count(a, kCntPair);
@@ -603,12 +654,22 @@ __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, i
   }
 }
 
-template <int GEOM>
+template <int GEOM, bool LDS>
 __global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES
 k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head, QueueView<R> qe,
           QueueView<R> qg) {
+  // a launch with less work than the grid holds lanes keeps only the blocks
+  // it needs (the others would find the input exhausted); an empty drain
+  // round costs one load of the queue count per block
+  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const int n_all = n_in + (int)(pr.end - pr.beg);
+  if ((int64_t)blockIdx.x * kBlock >= (int64_t)n_all) return;
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
+  __shared__ typename LdsLinOf<LDS>::type lin;
+  if constexpr (LDS) load_lds_linear(P.T, lin);   // made visible by acc_init's barrier
+  PTables<LDS> PT{P.T, nullptr};
+  if constexpr (LDS) PT.L = lin.v;
   const int wave = threadIdx.x >> 6;
   if (lane_id() == 0) {
     wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
@@ -617,8 +678,6 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   acc_init(acc, GEOM == GEOM_SLAB);
   const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
   const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
-  const int n_in = *in.count < in.cap ? *in.count : in.cap;
-  const int n_all = n_in + (int)(pr.end - pr.beg);
   Track t;
   double eprim = 0.0;
   bool has = false, exhausted = false;
@@ -639,9 +698,10 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
       if (exhausted) break;
       continue;
     }
+    count(acc, kCntLaneSlots);   // 64 per trip: the wave is converged here
     if (has) {
       t.rng.topup();
-      has = photon_step<GEOM>(acc, P, t, oe, og);
+      has = photon_step<GEOM, LDS>(acc, P, PT, t, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
   }
@@ -923,6 +983,10 @@ template <int GEOM, bool LDS>
 __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES
 k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head,
             QueueView<R> qe, QueueView<R> qg) {
+  // blocks beyond the launch's work leave at once (see k_photons)
+  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const int n_all = n_in + (int)(pr.end - pr.beg);
+  if ((int64_t)blockIdx.x * kBlock >= (int64_t)n_all) return;
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
   __shared__ typename LdsOf<LDS>::type lsp;
@@ -943,8 +1007,6 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
   acc_init(acc, GEOM == GEOM_SLAB);
   const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
   const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
-  const int n_in = *in.count < in.cap ? *in.count : in.cap;
-  const int n_all = n_in + (int)(pr.end - pr.beg);
   Track t;
   EState s;
   WComb wc{-1, 0.0};
@@ -966,6 +1028,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
       if (exhausted) break;
       continue;
     }
+    count(acc, kCntLaneSlots);   // 64 per trip: the wave is converged here
     if (has) {
       t.rng.topup();
       has = electron_step<GEOM, LDS>(acc, P, SP, t, s, wc, hot, wave, oe, og);
@@ -1091,10 +1154,20 @@ __global__ void k_eval(Tables<R> T, int which, int n, const double* e, const int
 hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int geom,
                                             int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,
                                             const QueueView<R>& qg, hipStream_t st) {
+#if DPM_PREC == 32
+  if (P.lds_photon) {
+    if (geom == GEOM_SLAB)
+      hipLaunchKernelGGL((k_photons<GEOM_SLAB, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    else
+      hipLaunchKernelGGL((k_photons<GEOM_VOXEL, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
+                         qg);
+    return hipGetLastError();
+  }
+#endif
   if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL(k_photons<GEOM_SLAB>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_photons<GEOM_SLAB, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   else
-    hipLaunchKernelGGL(k_photons<GEOM_VOXEL>, dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_photons<GEOM_VOXEL, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   return hipGetLastError();
 }
 

@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("e_primary", ctypes.c_double), ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64), ("counters", ctypes.c_uint64 * 10),
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
-                ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int), ("lds_tables", ctypes.c_int)]
+                ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int), ("lds_tables", ctypes.c_int),
+                ("lane_slots", ctypes.c_uint64 * 2)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
@@ -66,7 +67,8 @@ class Stats(ctypes.Structure):
                     counters_photon=dict(zip(COUNTER_NAMES, list(self.counters_photon))),
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
-                    kernel_calls=list(self.kernel_calls), graphs=self.graphs, lds_tables=self.lds_tables)
+                    kernel_calls=list(self.kernel_calls), graphs=self.graphs, lds_tables=self.lds_tables,
+                    lane_slots=list(self.lane_slots))
 
 
 class Schedule(ctypes.Structure):

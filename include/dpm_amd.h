@@ -166,10 +166,12 @@ typedef struct dpm_run_options {
                                       per-kernel timings in dpm_stats) */
   int rounds_per_sync;             /* transport rounds enqueued (or captured) per host synchronisation:
                                       even, 2..8; 0 = 8 */
-  int table_memory;                /* DPM_TABLES_AUTO (0): FP32 runs copy the electron spline tables
-                                      into each block's LDS when they fit (3*nmat+2 tables of 128
-                                      nodes: up to 3 materials); DPM_TABLES_GLOBAL (1): read them from
-                                      device memory. Both give identical results. */
+  int table_memory;                /* DPM_TABLES_AUTO (0): FP32 runs copy the small tables into each
+                                      block's LDS when they fit -- the electron spline tables
+                                      (3*nmat+2 tables of 128 nodes: up to 3 materials) and the photon
+                                      IMFP tables (2+3*nmat arrays of 1024 values: up to 2 materials);
+                                      DPM_TABLES_GLOBAL (1): read them from device memory. Both give
+                                      identical results. */
 } dpm_run_options;
 
 enum { DPM_TABLES_AUTO = 0, DPM_TABLES_GLOBAL = 1 };
@@ -193,7 +195,11 @@ typedef struct dpm_stats {
   uint64_t counters_photon[DPM_NCNT];    /* counters of the photon kernel */
   uint64_t counters_electron[DPM_NCNT];  /* counters of the electron kernel */
   int graphs;                      /* hipGraphs captured by the run (graph = 1) */
-  int lds_tables;                  /* 1: the electron kernel read its spline tables from LDS */
+  int lds_tables;                  /* bit 0: the electron kernel read its spline tables from LDS;
+                                      bit 1: the photon kernel its linear IMFP tables */
+  uint64_t lane_slots[2];          /* photon / electron kernels: 64 per transport-loop trip of a
+                                      wave; the steps (counters_*[DPM_CNT_G_STEPS / _E_STEPS]) over
+                                      this are the lane utilisation */
 } dpm_stats;
 
 /* Replaces Simulate(nprimary, eprimary, iselectron, lbox, mat, el, ph,
