@@ -1,0 +1,92 @@
+"""Configurations 3 and 5 at their full sizes, checked through properties that
+do not depend on the size (the FP64 oracle cannot run them in seconds; the
+reduced-size recipes of the same phantoms are compared with it history by
+history in test_gpu_transport.py):
+
+* energy conservation of the whole run: scored + escaped energy equals the
+  primaries' kinetic energy plus 2 mc^2 per annihilation minus 2 mc^2 per
+  pair production;
+* batch statistics: the batch sum of squares bounds the squared sum
+  (nbatch * sum2 >= sum^2 per voxel, Cauchy-Schwarz) and the batch sums add
+  up to the tally;
+* FP32 transport against FP64 transport of the same histories: 1 %/1 mm
+  gamma pass rate >= 99 % over voxels above 10 % of the maximum dose;
+* configuration 5: the mean sampled primary energy equals the mean of the
+  triangular (0.1, 2, 18) MeV spectrum within 5 standard errors.
+"""
+import numpy as np
+import pytest
+
+import dpm_amd
+import gamma
+import phantoms
+
+EMC2 = 0.510991
+
+
+def _balance(st, e_primary):
+    return (st["e_scored"] + st["e_escaped"]) - (e_primary + 2 * EMC2 * (st["n_annih"] - st["n_pair"]))
+
+
+def _batch_checks(dose, sum2, nbatch):
+    # dose is the sum over batches; sum2 the sum of squared batch tallies
+    assert np.all(sum2 >= -1e-300)
+    assert np.all(nbatch * sum2 * (1 + 1e-9) + 1e-300 >= dose * dose)
+
+
+@pytest.mark.gpu
+def test_config3_full_size(gpu_tables3):
+    # 256x256x128 voxels of 2 mm: water with 40 bone and 4 Ti ellipsoids and
+    # lognormal(0.05) density jitter; 12 MeV e- 10x10 cm2 field, 1e8 histories
+    mat, rho = phantoms.heterogeneous(256, 256, 128, 2.0, seed=1234)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 2.0)
+    n, e0, nbatch = 100_000_000, 12.0, 100
+    kw = dict(particle=-1, energy=e0, field=100.0, seed=3, nbatch=nbatch)
+    print("config 3 full size: FP32 run of %d histories" % n, flush=True)
+    d32, s32, st32, _ = dpm_amd.simulate(gpu_tables3, ph, n, **kw)
+    bal = _balance(st32, n * e0)
+    assert abs(bal) < 1e-5 * n * e0, bal
+    assert abs(d32.sum() - st32["e_scored"]) < 1e-9 * st32["e_scored"]
+    _batch_checks(d32, s32, nbatch)
+    print("config 3 full size: FP64 run", flush=True)
+    d64, _, st64, _ = dpm_amd.simulate(gpu_tables3, ph, n, precision=dpm_amd.DPM_FP64, **kw)
+    bal64 = _balance(st64, n * e0)
+    assert abs(bal64) < 1e-9 * n * e0, bal64
+    pr = gamma.gamma_pass_rate_3d(d64 / rho, d32 / rho, spacing_mm=2.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    print("config 3 full size: gamma FP32 vs FP64 %.5f" % pr, flush=True)
+    assert pr >= 0.99, pr
+    ph.close()
+
+
+@pytest.mark.gpu
+def test_config5_full_size(gpu_tables3):
+    # 512x512x256 voxels of 1 mm (the heterogeneous recipe, ellipsoids x2);
+    # 36 gantry fields every 10 degrees, 10x10 cm2, triangular (0.1, 2, 18)
+    # MeV photons, uneven history counts in [1e7, 5e7], 1e9 in total
+    mat, rho = phantoms.heterogeneous(512, 512, 256, 1.0, seed=2024, axis_scale=2.0)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 1.0)
+    plan = phantoms.config5_fields(1e9)
+    n = sum(f[3] for f in plan)
+    assert n == 1_000_000_000 and min(f[3] for f in plan) >= 1e7 and max(f[3] for f in plan) <= 5e7
+    kw = dict(particle=0, emin=0.1, emode=2.0, emax=18.0, field=100.0, seed=5)
+    print("config 5 full size: FP32 run of %d histories in %d fields" % (n, len(plan)), flush=True)
+    d32, s32, st32 = dpm_amd.simulate_fields(gpu_tables3, ph, plan, nbatch=100, **kw)
+    bal = _balance(st32, st32["e_primary"])
+    assert abs(bal) < 1e-5 * st32["e_primary"], bal
+    _batch_checks(d32, s32, 100)
+    mean_e = st32["e_primary"] / n
+    sd = np.sqrt((0.1 ** 2 + 2.0 ** 2 + 18.0 ** 2 - 0.1 * 2.0 - 0.1 * 18.0 - 2.0 * 18.0) / 18.0)
+    assert abs(mean_e - (0.1 + 2.0 + 18.0) / 3) < 5 * sd / np.sqrt(n), mean_e
+    # FP32 vs FP64 on a tenth of the plan (same angles and proportions),
+    # gamma over the central 128^3 mm^3 where the fields cross
+    sub = phantoms.config5_fields(1e8)
+    print("config 5 full size: FP32 and FP64 runs of 1e8 histories", flush=True)
+    a32, _, _ = dpm_amd.simulate_fields(gpu_tables3, ph, sub, **kw)
+    a64, _, st64 = dpm_amd.simulate_fields(gpu_tables3, ph, sub, precision=dpm_amd.DPM_FP64, **kw)
+    bal64 = _balance(st64, st64["e_primary"])
+    assert abs(bal64) < 1e-9 * st64["e_primary"], bal64
+    c = (slice(64, 192), slice(192, 320), slice(192, 320))
+    pr = gamma.gamma_pass_rate_3d(a64[c] / rho[c], a32[c] / rho[c], spacing_mm=1.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    print("config 5 full size: gamma FP32 vs FP64 (central 128^3) %.5f" % pr, flush=True)
+    assert pr >= 0.99, pr
+    ph.close()
