@@ -106,6 +106,144 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
                       "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
 
 
+def _oracle_voxel_worker(spec):
+    """One CPU-baseline process (bench.py --oracle-worker JSON): the FP64
+    oracle on a phantom saved as .npy; prints its wall time. Touches no GPU."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # the checker, timed as the CPU baseline
+    a = json.loads(spec)
+    mat = np.load(a["mat"], mmap_mode="r")
+    rho = np.load(a["rho"], mmap_mode="r")
+    oracle_lib.load(a["tables"])   # table loading stays outside the timed region
+    t0 = time.perf_counter()
+    oracle_lib.run_voxel(a["tables"], mat, rho, a["lbox"], **a["kw"])
+    print(json.dumps({"seconds": time.perf_counter() - t0}), flush=True)
+
+
+def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=4000):
+    """The FP64 oracle on this host's cores (<= 16 processes) over `jobs`, a
+    list of run_voxel keyword sets (history ranges of the workload); each
+    process runs about seconds_target."""
+    import tempfile
+    import numpy as np
+    cores = min(16, os.cpu_count() or 1)
+    d = tempfile.mkdtemp(prefix="dpm_cpu_")
+    mp_, rp_ = os.path.join(d, "mat.npy"), os.path.join(d, "rho.npy")
+    np.save(mp_, mat)
+    np.save(rp_, rho)
+
+    def spawn(kw):
+        spec = json.dumps({"tables": tables, "mat": mp_, "rho": rp_, "lbox": lbox, "kw": kw})
+        return subprocess.Popen([sys.executable, os.path.abspath(__file__), "--oracle-worker", spec],
+                                stdout=subprocess.PIPE, text=True)
+
+    def seconds(p):
+        out = p.communicate()[0]
+        if p.returncode != 0:
+            raise RuntimeError("oracle worker failed")
+        return json.loads(out.strip().splitlines()[-1])["seconds"]
+
+    try:
+        sec = seconds(spawn(dict(jobs[0], nhist=cal)))
+        n_each = max(cal, int(cal / sec * seconds_target))
+        # disjoint history ranges: process i takes the (i // len(jobs))-th slice of job i % len(jobs)
+        t0 = time.perf_counter()
+        procs = [spawn(dict(jobs[i % len(jobs)], nhist=n_each,
+                            hist_begin=jobs[i % len(jobs)].get("hist_begin", 0) + (i // len(jobs)) * n_each))
+                 for i in range(cores)]
+        secs = [seconds(p) for p in procs]
+        wall = time.perf_counter() - t0
+    finally:
+        for f in (mp_, rp_):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+    # rate over the slowest process's transport time (Python start-up and
+    # table loading excluded, so the CPU is not understated)
+    return {"value": cores * n_each / max(secs), "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": "%d concurrent processes x %d histories (FP64 oracle, Philox mode), slowest %.1f s "
+                      "(wall with start-up %.1f s); single-core %.0f hist/s"
+                      % (cores, n_each, max(secs), wall, n_each / (sum(secs) / cores))}
+
+
+def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
+    """Configuration 3: 256x256x128 heterogeneous phantom at 2 mm, 12 MeV e-
+    10x10 cm2 field, 1e8 histories, FP32 transport, FP64 tally, 100 batches."""
+    import numpy as np
+    t = dpm_amd.Tables(tables3)
+    mat, rho = phantoms.heterogeneous(256, 256, 128, 2.0, seed=1234)
+    ph = dpm_amd.Phantom.voxel(t, mat, rho, 2.0)
+    n, nb = 100_000_000, 100
+    kw = dict(particle=dpm_amd.ELECTRON, energy=12.0, field=100.0, seed=3, nbatch=nb)
+    sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0, nbatch=nb)
+    sched["chunk"] = 0
+    out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
+    dpm_amd.simulate(t, ph, 1_000_000, schedule=sched, out=out, fetch=False, **kw)   # warm-up
+    t0 = time.perf_counter()
+    _, _, st, _ = dpm_amd.simulate(t, ph, n, schedule=sched, out=out, fetch=False, **kw)
+    wall = time.perf_counter() - t0
+    d32, _ = dpm_amd.tally_fetch(t, ph, out=out)
+    d32 = d32.copy()
+    d64, _, st64, _ = dpm_amd.simulate(t, ph, n, precision=dpm_amd.DPM_FP64, **kw)
+    pr = gamma.gamma_pass_rate_3d(d64 / rho, d32 / rho, spacing_mm=2.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    r = {"workload": "config3: 256x256x128 heterogeneous phantom (40 bone + 4 Ti ellipsoids, density x "
+                     "lognormal(0.05)), 2 mm, 12 MeV e- 10x10 cm2 field, 1e8 histories, FP32 transport, FP64 "
+                     "tally, 100 batches",
+         "histories_per_s": n / wall, "seconds": wall, "schedule": sched,
+         "gamma_fp32_vs_fp64": {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max, full phantom",
+                                "fp64_histories_per_s": n / st64["seconds"]}}
+    if cpu:
+        r["cpu_baseline"] = cpu_baseline_voxel(tables3, mat, rho, 2.0, [dict(ptype=-1, e0=12.0, field=100.0,
+                                                                              seed=3, hist_begin=10_000_000)])
+        r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
+    ph.close()
+    t.close()
+    return r
+
+
+def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
+    """Configuration 5: 512x512x256 heterogeneous phantom at 1 mm, 36 gantry
+    photon fields (10x10 cm2, triangular 0.1/2/18 MeV, uneven counts), 1e9
+    histories in one run, FP32 transport, FP64 tally, 100 batches."""
+    import numpy as np
+    t = dpm_amd.Tables(tables3)
+    mat, rho = phantoms.heterogeneous(512, 512, 256, 1.0, seed=2024, axis_scale=2.0)
+    ph = dpm_amd.Phantom.voxel(t, mat, rho, 1.0)
+    plan = phantoms.config5_fields(1e9)
+    n = sum(f[3] for f in plan)
+    kw = dict(particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0, field=100.0, seed=5)
+    # schedule tuned on one field of the plan (the 45-degree beam, whose rays cross the most voxels)
+    sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
+                             field=100.0, angle=45.0, nbatch=100)
+    sched["chunk"] = 0
+    dpm_amd.simulate_fields(t, ph, phantoms.config5_fields(1e7), fetch=False, schedule=sched, **kw)   # warm-up
+    t0 = time.perf_counter()
+    _, _, st = dpm_amd.simulate_fields(t, ph, plan, nbatch=100, fetch=False, schedule=sched, **kw)
+    wall = time.perf_counter() - t0
+    sub = phantoms.config5_fields(1e8)
+    a32, _, _ = dpm_amd.simulate_fields(t, ph, sub, **kw)
+    a64, _, st64 = dpm_amd.simulate_fields(t, ph, sub, precision=dpm_amd.DPM_FP64, **kw)
+    c = (slice(64, 192), slice(192, 320), slice(192, 320))
+    pr = gamma.gamma_pass_rate_3d(a64[c] / rho[c], a32[c] / rho[c], spacing_mm=1.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    r = {"workload": "config5: 512x512x256 heterogeneous phantom at 1 mm, 36 gantry photon fields every 10 deg, "
+                     "10x10 cm2, triangular (0.1, 2, 18) MeV, uneven counts in [1e7, 5e7], 1e9 histories in one "
+                     "run, FP32 transport, FP64 tally, 100 batches",
+         "histories_per_s": n / wall, "seconds": wall, "schedule": sched,
+         "gamma_fp32_vs_fp64": {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max, 1e8 histories "
+                                                              "of the plan, central 128^3 voxels",
+                                "fp64_histories_per_s": sum(f[3] for f in sub) / st64["seconds"]}}
+    if cpu:
+        offs = np.concatenate([[0], np.cumsum([f[3] for f in plan])])
+        jobs = [dict(ptype=0, emin=0.1, emode=2.0, emax=18.0, field=100.0, angle=plan[i][2], seed=5,
+                     hist_begin=int(offs[i])) for i in range(len(plan))]
+        r["cpu_baseline"] = cpu_baseline_voxel(tables3, mat, rho, 1.0, jobs)
+        r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
+    ph.close()
+    t.close()
+    return r
+
+
 def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, sched=None, plain=True):
     """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
     per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
@@ -172,9 +310,15 @@ def main():
     ap.add_argument("--refill-min", type=int, default=0, help="fix the refill threshold (skips the autotuner)")
     ap.add_argument("--tally-mode", type=int, default=-1, help="fix the voxel-tally mode (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="beamlet leg: skip the plain per-beamlet runs")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the configuration 3 and 5 legs (full-size heterogeneous runs)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device-to-device copy of this many GiB (counter calibration)")
+    ap.add_argument("--oracle-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.oracle_worker is not None:
+        _oracle_voxel_worker(args.oracle_worker)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU split (history ranges per GPU, one FP64 "
               "all-reduce of the dose) is not built, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -219,8 +363,9 @@ def main():
                  "tally_mode": max(0, args.tally_mode), "source": "command line / defaults"}
     else:
         ns = min(nhist, 1 << 24)
-        sched = dpm_amd.autotune(t, ph, nsample=ns, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field)
-        sched["source"] = "dpm_autotune on %d histories of this workload" % ns
+        sched = dpm_amd.autotune(t, ph, nsample=ns, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field,
+                                 nbatch=args.nbatch)
+        sched["source"] = "dpm_autotune on %d histories of this workload (%d batches)" % (ns, args.nbatch)
         sched["chunk"] = 0
 
     def step(precision=dpm_amd.DPM_FP32, seed=1, fetch=True):
@@ -292,6 +437,16 @@ def main():
                                          sched=None if not fixed else sched, plain=not args.no_plain)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
+    if not args.no_extra:
+        import phantoms
+        ph.close()
+        t.close()
+        tables3 = __graft_entry__.tables_dir(3)
+        for key, leg in (("config3", config3_leg), ("config5", config5_leg)):
+            try:
+                result[key] = leg(dpm_amd, gamma, phantoms, tables3, cpu=not args.no_cpu)
+            except Exception as e:   # a failing extra leg is reported, the headline line still prints
+                result[key] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(result), flush=True)
 
 

@@ -1378,11 +1378,19 @@ namespace {
 // Times every candidate schedule on the sample source `hs` (device time of
 // run_transport, best of two runs) and writes the fastest to *out.
 template <typename R>
-int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int rows, Workspace<R>& ws,
+int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int rows, int nbatch, Workspace<R>& ws,
                   Tables<R>& T, dpm_schedule* out) {
   const uint64_t n = hs.count(0) + hs.count(1);
   const int64_t nb = p->geom == GEOM_SLAB ? p->nbins : p->nvox;
-  if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)))
+  // batch statistics as the real run keeps them: the batch rows of a chunk
+  // (their footprint sets the cost of the tally atomics) and the fold
+  if (p->geom == GEOM_SLAB || nbatch < 2 || (uint64_t)nbatch > n) nbatch = 0;
+  if (nbatch > 0) {
+    const uint64_t per = n / nbatch + 1;
+    rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)kDefaultChunk / per));
+  }
+  if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)) ||
+      (nbatch > 0 && (!t->buf_sum.ensure(sizeof(double) * nb) || !t->buf_sum2.ensure(sizeof(double) * nb))))
     return set_error(DPM_E_NOMEM, "cannot allocate the tally");
   static const int kWaves[] = {8, 12, 16};
   static const int kRefill[] = {1, 4, 16};
@@ -1400,6 +1408,7 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
         opt.seed = 0x5eed;
         opt.hist_begin = 0;
         opt.chunk = (int64_t)n;
+        opt.nbatch = nbatch;
         opt.waves_per_cu = w;
         opt.refill_min = r;
         opt.tally_mode = tm;
@@ -1411,7 +1420,8 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
           dpm_stats st;
           std::memset(&st, 0, sizeof(st));
           const int rc = run_transport<R>(t, p, hs, &opt, ws, T, t->buf_rows.d(), rows,
-                                          p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr, nullptr, nullptr,
+                                          p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr,
+                                          nbatch > 0 ? t->buf_sum.d() : nullptr, nbatch > 0 ? t->buf_sum2.d() : nullptr,
                                           nullptr, &st);
           if (rc != DPM_OK) return rc;
           sec = std::min(sec, st.seconds);
@@ -1438,7 +1448,7 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
 extern "C" {
 
 int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nsample, int precision,
-                 dpm_schedule* best) {
+                 int nbatch, dpm_schedule* best) {
   if (!t || !p || !src || !best) return set_error(DPM_E_INVALID, "null argument");
   if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
   if (nsample < 1000 || nsample > ((uint64_t)1 << 24)) return set_error(DPM_E_INVALID, "sample of 1e3 .. 2^24 histories");
@@ -1447,8 +1457,8 @@ int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   if (rc != DPM_OK) return rc;
   HIP_TRY(hipSetDevice(t->device));
   const HostSource hs = source_from(src, nsample, 0);
-  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, 1, t->wsd, t->td, best)
-                               : autotune_impl<float>(t, p, hs, 1, t->wsf, t->tf, best);
+  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, 1, nbatch, t->wsd, t->td, best)
+                               : autotune_impl<float>(t, p, hs, 1, nbatch, t->wsf, t->tf, best);
 }
 
 int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb, uint64_t nsample,
@@ -1475,8 +1485,8 @@ int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
     first += n;
   }
   HIP_TRY(hipSetDevice(t->device));
-  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, nb, t->wsd, t->td, best)
-                               : autotune_impl<float>(t, p, hs, nb, t->wsf, t->tf, best);
+  return precision == DPM_FP64 ? autotune_impl<double>(t, p, hs, nb, 0, t->wsd, t->td, best)
+                               : autotune_impl<float>(t, p, hs, nb, 0, t->wsf, t->tf, best);
 }
 
 int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val) {

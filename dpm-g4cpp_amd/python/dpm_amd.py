@@ -117,7 +117,7 @@ def lib():
         L.dpm_simulate_fields.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.POINTER(Field), ip,
                                           ctypes.POINTER(RunOptions), vp, vp, ctypes.POINTER(Stats)]
         L.dpm_perform.argtypes = [vp, ip, ip, ip, dp, ip, ctypes.c_uint64, vp, vp]
-        L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ctypes.POINTER(Schedule)]
+        L.dpm_autotune.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64, ip, ip, ctypes.POINTER(Schedule)]
         L.dpm_autotune_beamlets.argtypes = [vp, vp, ctypes.POINTER(Beamlet), ip, ctypes.c_uint64, ip,
                                             ctypes.POINTER(Schedule)]
         L.dpm_host_free.argtypes = [vp]
@@ -337,11 +337,13 @@ def tally_fetch(tables, phantom, out=None):
 
 
 def autotune(tables, phantom, nsample=2000000, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0,
-             x0=0.0, y0=0.0, precision=DPM_FP32, angle=None, emode=None):
-    """Fastest launch schedule for this source on this phantom (dpm_autotune); a dict usable as schedule=."""
+             x0=0.0, y0=0.0, precision=DPM_FP32, angle=None, emode=None, nbatch=0):
+    """Fastest launch schedule for this source on this phantom (dpm_autotune); a dict usable as schedule=.
+    nbatch: time the candidates with the batch statistics of a run with that many batches."""
     src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode)
     sch = Schedule()
-    _check(lib().dpm_autotune(tables.h, phantom.h, ctypes.byref(src), int(nsample), precision, ctypes.byref(sch)))
+    _check(lib().dpm_autotune(tables.h, phantom.h, ctypes.byref(src), int(nsample), precision, int(nbatch),
+                              ctypes.byref(sch)))
     return sch.as_dict()
 
 

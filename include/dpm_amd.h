@@ -275,8 +275,10 @@ typedef struct dpm_schedule {
   double default_histories_per_s;  /* device rate of the default schedule on the sample */
 } dpm_schedule;
 
+/* nbatch (> 1, voxel phantoms): time the candidates with the batch
+ * statistics of a run with that many batches (their tally rows and folds). */
 int dpm_autotune(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nsample, int precision,
-                 dpm_schedule* best);
+                 int nbatch, dpm_schedule* best);
 int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb, uint64_t nsample,
                           int precision, dpm_schedule* best);
 

@@ -194,6 +194,9 @@ def test_autotune_returns_a_timed_candidate(gpu_tables0):
     assert sb["candidates"] == 27 and sb["histories_per_s"] > 0
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.autotune(gpu_tables0, ph, nsample=10, particle=0, emin=0.5, emax=20.0)
+    # with the batch rows and folds of a 10-batch run
+    s10 = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0, nbatch=10)
+    assert s10["candidates"] == 27 and s10["histories_per_s"] >= s10["default_histories_per_s"] > 0
 
 
 @pytest.mark.gpu
