@@ -510,6 +510,10 @@ struct dpm_tables {
   hipEvent_t ev[10 + 4 * kRoundsPerSync] = {};  // [10 + 4k ..]: kernel bounds of round k of a sync block
   // tally buffers reused across runs
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
+  // leading bytes of buf_rows known to be zero (left so by the folds of a
+  // completed batch run), valid while buf_rows still holds rows_zero_ptr
+  size_t rows_zero = 0;
+  const void* rows_zero_ptr = nullptr;
   DevBuf buf_src;            // source description of the current run
   DevBuf buf_blaux;          // beamlet row maxima, segment counts and offsets
   // result of the last dpm_simulate, kept on the device for dpm_tally_fetch
@@ -1230,7 +1234,12 @@ static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource
   double* d_sum = nbatch > 0 ? t->buf_sum.d() : nullptr;
   double* d_sum2 = nbatch > 0 ? t->buf_sum2.d() : nullptr;
   double* d_he = opt.hist_edep ? t->buf_he.d() : nullptr;
-  (void)hipMemsetAsync(d_rows, 0, sizeof(double) * nb * rows, s);
+  // a batch run's folds clear every row they use, so a following batch run
+  // finds its rows zero and skips clearing them again
+  const size_t rows_bytes = sizeof(double) * nb * rows;
+  const bool rows_clean = nbatch > 0 && t->rows_zero_ptr == t->buf_rows.p && t->rows_zero >= rows_bytes;
+  t->rows_zero = 0;
+  if (!rows_clean) (void)hipMemsetAsync(d_rows, 0, rows_bytes, s);
   if (d_steps) (void)hipMemsetAsync(d_steps, 0, sizeof(double) * nb, s);
   if (d_sum) (void)hipMemsetAsync(d_sum, 0, sizeof(double) * nb, s);
   if (d_sum2) (void)hipMemsetAsync(d_sum2, 0, sizeof(double) * nb, s);
@@ -1248,6 +1257,10 @@ static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource
     if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
   }
   if (rc == DPM_OK) rc = finish_stats(t, &local);
+  if (rc == DPM_OK && nbatch > 0) {
+    t->rows_zero = rows_bytes;
+    t->rows_zero_ptr = t->buf_rows.p;
+  }
   // the result stays on the device until the next run (dpm_tally_fetch)
   t->last_tally = rc == DPM_OK ? (nbatch > 0 ? d_sum : d_rows) : nullptr;
   t->last_second = rc == DPM_OK ? (slab ? d_steps : d_sum2) : nullptr;
@@ -1308,6 +1321,7 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   t->last_n = 0;
   const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
   const size_t nseg = (size_t)nb * kCsrSeg;
+  t->rows_zero = 0;   // the beamlet rows are cleared by this run, not left zero
   if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * nseg + 1024))
     return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string(rows_bytes >> 20) + " MiB)");
   double* d_rows = t->buf_rows.d();
@@ -1389,6 +1403,7 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
     const uint64_t per = n / nbatch + 1;
     rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)kDefaultChunk / per));
   }
+  t->rows_zero = 0;   // candidates leave their tallies in the rows
   if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)) ||
       (nbatch > 0 && (!t->buf_sum.ensure(sizeof(double) * nb) || !t->buf_sum2.ensure(sizeof(double) * nb))))
     return set_error(DPM_E_NOMEM, "cannot allocate the tally");

@@ -197,8 +197,10 @@ struct HotTable {
   unsigned long long key[kBlock / 64][kHotSlots];
   double val[kBlock / 64][kHotSlots];
 };
-template <int GEOM> struct HotOf { using type = HotTable; };
-template <> struct HotOf<GEOM_SLAB> { using type = NoLds; };
+// only the kernel variant launched for tally mode 2 holds the table (16 KB of
+// LDS per block would otherwise cost the other variants a block per CU)
+template <bool HOT> struct HotOf { using type = NoLds; };
+template <> struct HotOf<true> { using type = HotTable; };
 
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
@@ -979,7 +981,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   return true;
 }
 
-template <int GEOM, bool LDS>
+template <int GEOM, bool LDS, bool HOT>
 __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES
 k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head,
             QueueView<R> qe, QueueView<R> qg) {
@@ -997,9 +999,8 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     SP.off = lds_spline_offsets(P.T);
   }
   const int wave = threadIdx.x >> 6;
-  __shared__ typename HotOf<GEOM>::type hot;
-  if constexpr (GEOM == GEOM_VOXEL)
-    if (P.tally_mode == 2) hot_init(hot, wave);
+  __shared__ typename HotOf<HOT>::type hot;
+  if constexpr (HOT) hot_init(hot, wave);
   if (lane_id() == 0) {
     wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
     wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
@@ -1036,8 +1037,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
       if (!has) flush_wc(acc, P, wc, hot, wave);
     }
   }
-  if constexpr (GEOM == GEOM_VOXEL)
-    if (P.tally_mode == 2) hot_flush(acc, P, hot, wave);
+  if constexpr (HOT) hot_flush(acc, P, hot, wave);
   flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
@@ -1171,27 +1171,29 @@ hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, const SourcePar
   return hipGetLastError();
 }
 
+template <int GEOM, bool LDS, bool HOT>
+static hipError_t launch_el(const KParams<R>& P, const SourceParams* S, const PrimRange& pr, int blocks,
+                            const QueueView<R>& in, int* head, const QueueView<R>& qe, const QueueView<R>& qg,
+                            hipStream_t st) {
+  hipLaunchKernelGGL((k_electrons<GEOM, LDS, HOT>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+  return hipGetLastError();
+}
+
 hipError_t DPM_CAT(launch_electrons, DPM_SFX)(const KParams<R>& P, const SourceParams* S, const PrimRange& pr,
                                               int geom, int blocks, const QueueView<R>& in, int* head,
                                               const QueueView<R>& qe, const QueueView<R>& qg, hipStream_t st) {
+  // the slab tally is privatised in LDS by depth bin; tally mode 2 is for voxels
+  const bool hot = geom == GEOM_VOXEL && P.tally_mode == 2;
 #if DPM_PREC == 32
   if (P.lds_tables) {
-    if (geom == GEOM_SLAB)
-      hipLaunchKernelGGL((k_electrons<GEOM_SLAB, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
-                         qg);
-    else
-      hipLaunchKernelGGL((k_electrons<GEOM_VOXEL, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
-                         qg);
-    return hipGetLastError();
+    if (geom == GEOM_SLAB) return launch_el<GEOM_SLAB, true, false>(P, S, pr, blocks, in, head, qe, qg, st);
+    return hot ? launch_el<GEOM_VOXEL, true, true>(P, S, pr, blocks, in, head, qe, qg, st)
+               : launch_el<GEOM_VOXEL, true, false>(P, S, pr, blocks, in, head, qe, qg, st);
   }
 #endif
-  if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL((k_electrons<GEOM_SLAB, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
-                       qg);
-  else
-    hipLaunchKernelGGL((k_electrons<GEOM_VOXEL, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
-                       qg);
-  return hipGetLastError();
+  if (geom == GEOM_SLAB) return launch_el<GEOM_SLAB, false, false>(P, S, pr, blocks, in, head, qe, qg, st);
+  return hot ? launch_el<GEOM_VOXEL, false, true>(P, S, pr, blocks, in, head, qe, qg, st)
+             : launch_el<GEOM_VOXEL, false, false>(P, S, pr, blocks, in, head, qe, qg, st);
 }
 
 hipError_t DPM_CAT(launch_perform, DPM_SFX)(const Tables<R>& T, int kind, int n, double energy, int imat,

@@ -421,3 +421,26 @@ def test_atomic_probe_counts_every_addition(gpu_tables0):
     assert nops < total <= nops + 4096 * 1024 and total == int(total)
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.probe_atomics(gpu_tables0, 1 << 32, 10)
+
+
+@pytest.mark.gpu
+def test_batch_rows_reused_clean_across_runs(gpu_tables0):
+    # a batch run leaves its rows cleared by the folds and the next batch run
+    # skips clearing them; runs of other kinds in between invalidate that
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    y = dict(particle=0, emin=0.5, emax=20.0, field=40.0, seed=31, nbatch=8)
+    d1, q1, _, _ = dpm_amd.simulate(gpu_tables0, ph, 20000, **y)
+    for other in (dict(particle=-1, energy=9.0, seed=5, nbatch=5),      # batch run of another source
+                  dict(particle=-1, energy=9.0, seed=6),                # single-row run: tally left in row 0
+                  None):                                                # beamlets
+        if other is None:
+            dpm_amd.simulate_beamlets(gpu_tables0, ph, [(0.0, 0.0, -1, 9.0, 3000), (8.0, 2.0, 0, 5.0, 3000)])
+        else:
+            dpm_amd.simulate(gpu_tables0, ph, 7000, **other)
+        d2, q2, _, _ = dpm_amd.simulate(gpu_tables0, ph, 20000, **y)
+        np.testing.assert_allclose(d2, d1, rtol=1e-11, atol=1e-12 * d1.max())
+        np.testing.assert_allclose(q2, q1, rtol=1e-11, atol=1e-12 * q1.max())
+        d3, q3, _, _ = dpm_amd.simulate(gpu_tables0, ph, 20000, **y)   # straight after a batch run
+        np.testing.assert_allclose(d3, d1, rtol=1e-11, atol=1e-12 * d1.max())
+        np.testing.assert_allclose(q3, q1, rtol=1e-11, atol=1e-12 * q1.max())
