@@ -81,29 +81,57 @@ def stream_copy_gbs(gib=1.0, reps=3):
     return 2.0 * n * 4 / best / 1e9
 
 
-def cpu_baseline(tables0, nx, lbox, field, emin, emax, seconds_target=15.0):
+def cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, seed, gpu_dose=None, gamma=None):
+    """The FP64 oracle CLI on this host's cores (<= 16 processes) over exactly
+    the histories of one timed step (seed, ids [0, nhist) split into disjoint
+    ranges): the CPU rate, and the CPU dose of the same histories for an
+    equal-history comparison with the GPU's FP32 dose."""
+    import tempfile
+    import numpy as np
     import oracle_lib  # the checker's CLI, timed as the CPU baseline
     cli = oracle_lib.CLI_PATH
     if not os.path.exists(cli):
         return None
     cores = min(16, os.cpu_count() or 1)
-    args = [cli, "--data", tables0, "--mode", "philox", "--geom", "voxel", "--nx", str(nx), "--lbox", str(lbox),
-            "--ptype", "0", "--emin", str(emin), "--emax", str(emax), "--field", str(field)]
-    # calibrate on one core
-    out = subprocess.run(args + ["--n", "2000", "--hist-begin", "0"], capture_output=True, text=True, check=True)
-    rate1 = json.loads(out.stdout.strip().splitlines()[-1])["hist_per_s"]
-    n_each = max(2000, int(rate1 * seconds_target))
-    t0 = time.perf_counter()
-    procs = [subprocess.Popen(args + ["--n", str(n_each), "--hist-begin", str(1000000 + i * n_each)],
-                              stdout=subprocess.PIPE, text=True) for i in range(cores)]
-    outs = [p.communicate()[0] for p in procs]
-    wall = time.perf_counter() - t0
-    if any(p.returncode != 0 for p in procs):
-        return None
-    per = [json.loads(o.strip().splitlines()[-1]) for o in outs]
-    return {"value": cores * n_each / wall, "unit": "histories/s", "cores": cores, "kind": "port",
-            "sample": "%d processes x %d histories of the same workload (FP64 oracle, Philox mode), wall %.1f s; "
-                      "single-core %.0f hist/s" % (cores, n_each, wall, sum(p["hist_per_s"] for p in per) / cores)}
+    args = [cli, "--data", tables0, "--mode", "philox", "--seed", str(seed), "--geom", "voxel", "--nx", str(nx),
+            "--lbox", str(lbox), "--ptype", "0", "--emin", str(emin), "--emax", str(emax), "--field", str(field)]
+    d = tempfile.mkdtemp(prefix="dpm_cpu_")
+    files = [os.path.join(d, "dose%d.bin" % i) for i in range(cores)]
+    bounds = [nhist * i // cores for i in range(cores + 1)]
+    try:
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen(args + ["--n", str(bounds[i + 1] - bounds[i]), "--hist-begin", str(bounds[i]),
+                                          "--dose", files[i]], stdout=subprocess.PIPE, text=True)
+                 for i in range(cores)]
+        outs = [p.communicate()[0] for p in procs]
+        wall = time.perf_counter() - t0
+        if any(p.returncode != 0 for p in procs):
+            return None
+        per = [json.loads(o.strip().splitlines()[-1]) for o in outs]
+        cpu_dose = None
+        if gpu_dose is not None:
+            cpu_dose = np.zeros(nx ** 3)
+            for f in files:
+                cpu_dose += np.fromfile(f, dtype=np.float64)
+    finally:
+        for f in files:
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+    r = {"value": nhist / wall, "unit": "histories/s", "cores": cores, "kind": "port",
+         "sample": "the timed step's %d histories (seed %d) in %d processes with disjoint history ranges (FP64 "
+                   "oracle, Philox mode), wall %.1f s; single-core %.0f hist/s"
+                   % (nhist, seed, cores, wall, sum(p["hist_per_s"] for p in per) / cores)}
+    if cpu_dose is not None:
+        ref = cpu_dose.reshape(gpu_dose.shape)
+        m = ref >= 0.1 * ref.max()
+        r["equal_history_dose"] = {
+            "gamma_pass_rate_gpu_fp32_vs_cpu": gamma.gamma_pass_rate_3d(ref, gpu_dose, spacing_mm=lbox, dd=0.01,
+                                                                         dta_mm=1.0, cutoff=0.1),
+            "max_abs_diff_above_cutoff_rel_to_max": float(np.max(np.abs(gpu_dose[m] - ref[m])) / ref.max()),
+            "total_energy_rel_diff": float(abs(gpu_dose.sum() - ref.sum()) / ref.sum()),
+            "criterion": "1%/1mm global, voxels > 10% of the CPU dose maximum; same histories and random streams"}
+    return r
 
 
 def _oracle_voxel_worker(spec):
@@ -436,7 +464,8 @@ def main():
         result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets,
                                          sched=None if not fixed else sched, plain=not args.no_plain)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax)
+        result["cpu_baseline"] = cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, 1, gpu_dose=dose,
+                                              gamma=gamma)
     if not args.no_extra:
         import phantoms
         ph.close()

@@ -12,6 +12,8 @@
 //   --e0 E --emin A --emax B --field F   (field source when F>0)
 //   --hist-begin H --n N
 //   --hist FILE       write the reference hist.sim (slab only)
+//   --dose FILE       write the voxel energy tally (MeV per voxel, float64,
+//                     [nz][ny][nx], raw binary; voxel only)
 // Prints one JSON line with the wall time and histories/s.
 #include "dpm_oracle.h"
 
@@ -24,7 +26,7 @@
 #include <vector>
 
 int main(int argc, char** argv) {
-  std::string data = "./data", mode = "philox", geom = "slab", hist;
+  std::string data = "./data", mode = "philox", geom = "slab", hist, dose;
   double lbox = 1.0, e0 = 15.0, emin = 0, emax = 0, field = 0, rho = 1.0;
   int gidx = 3, nx = 0, ptype = -1;
   unsigned long long seed = 1, hbegin = 0, n = 1000;
@@ -47,6 +49,7 @@ int main(int argc, char** argv) {
     else if (k == "--hist-begin") hbegin = std::strtoull(v, nullptr, 10);
     else if (k == "--n") n = std::strtoull(v, nullptr, 10);
     else if (k == "--hist") hist = v;
+    else if (k == "--dose") dose = v;
     else { std::fprintf(stderr, "unknown flag %s\n", k.c_str()); return 2; }
   }
   if (oracle_load(data.c_str()) != 0) { std::fprintf(stderr, "load: %s\n", oracle_error()); return 1; }
@@ -102,6 +105,15 @@ int main(int argc, char** argv) {
         std::fprintf(f, " %10d    %13.4e    %13.4e    %13.4e\n", i, (i + 0.5) * lbox * 0.1, edep[i] * norm, steps[i] / n);
       std::fclose(f);
     }
+  }
+  if (!dose.empty() && cfg.geom_mode == ORACLE_GEOM_VOXEL) {
+    FILE* f = std::fopen(dose.c_str(), "wb");
+    if (!f || std::fwrite(edep.data(), sizeof(double), edep.size(), f) != edep.size()) {
+      std::fprintf(stderr, "cannot write %s\n", dose.c_str());
+      if (f) std::fclose(f);
+      return 1;
+    }
+    std::fclose(f);
   }
   std::printf("{\"histories\": %llu, \"seconds\": %.6f, \"hist_per_s\": %.3f, \"e_scored\": %.9e, \"e_escaped\": %.9e}\n",
               n, dt, n / dt, out.e_scored, out.e_escaped);
