@@ -149,6 +149,16 @@ __device__ __forceinline__ void count(BlockAcc& a, int which) {
   const uint64_t act = __ballot(1);
   if (lane_id() == __ffsll((long long)act) - 1) atomicAdd(&a.cnt[which], (unsigned)__popcll(act));
 }
+// two counters that every active lane increments (one ballot)
+template <class BlockAcc>
+__device__ __forceinline__ void count2(BlockAcc& a, int w1, int w2) {
+  const uint64_t act = __ballot(1);
+  if (lane_id() == __ffsll((long long)act) - 1) {
+    const unsigned n = (unsigned)__popcll(act);
+    atomicAdd(&a.cnt[w1], n);
+    atomicAdd(&a.cnt[w2], n);
+  }
+}
 
 // Geom::Score (Geom.cc:120-130) for the slab; voxel tally row `tag`
 template <int GEOM, class BlockAcc>
@@ -619,6 +629,15 @@ template <int GEOM, class BlockAcc>
 __device__ __forceinline__ bool start_track(BlockAcc& a, const KParams<R>& P, Track& t) {
   if (t.type == (int)kEmptySlot) return false;
   if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) box_of<R, GEOM>(P.G, t.pos, t.box);
+  if constexpr (GEOM == GEOM_VOXEL) {
+    // every voxel holds a valid material (checked when the phantom was
+    // created), so the material lookup fails exactly outside the phantom:
+    // a bounds test gives the same answer without a dependent load
+    const bool inside = t.box[0] >= 0 && t.box[1] >= 0 && t.box[2] >= 0 && t.box[0] < P.G.nx &&
+                        t.box[1] < P.G.ny && t.box[2] < P.G.nz;
+    if (!inside) { escape(a, t.E); return false; }
+    return true;
+  }
   count(a, kCntVoxelVisits);
   const int imat = material_index<R, GEOM>(P.G, t.box);
   if (imat < 0) { escape(a, t.E); return false; }
@@ -684,6 +703,7 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   double eprim = 0.0;
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
+  uint32_t trips = 0;   // transport-loop trips of this wave (wave-uniform)
   for (;;) {
     // compaction interval: refill once refill_min lanes are idle (or none is busy)
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
@@ -700,13 +720,14 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
       if (exhausted) break;
       continue;
     }
-    count(acc, kCntLaneSlots);   // 64 per trip: the wave is converged here
+    ++trips;   // the wave is converged here
     if (has) {
       t.rng.topup();
       has = photon_step<GEOM, LDS>(acc, P, PT, t, oe, og);
       if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
     }
   }
+  if (lane_id() == 0) atomicAdd(&acc.cnt[kCntLaneSlots], 64u * trips);
   flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
@@ -816,8 +837,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
   // distance to boundary at the start of every step (SimDPMLike.cc:257, 451)
   const R stepGeom = distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
   if (stepGeom < (R)0.0) { escape(a, t.E); return false; }
-  count(a, kCntESteps);
-  count(a, kCntVoxelVisits);
+  count2(a, kCntESteps, kCntVoxelVisits);
   int imat = material_index<R, GEOM>(P.G, t.box);
   if (imat < 0) { escape(a, t.E); return false; }
   const R ecut = T.ecut;
@@ -1014,6 +1034,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
   double eprim = 0.0;
   bool has = false, exhausted = false;
   InRange rng_in{0, 0};
+  uint32_t trips = 0;   // transport-loop trips of this wave (wave-uniform)
   for (;;) {
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
@@ -1029,7 +1050,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
       if (exhausted) break;
       continue;
     }
-    count(acc, kCntLaneSlots);   // 64 per trip: the wave is converged here
+    ++trips;   // the wave is converged here
     if (has) {
       t.rng.topup();
       has = electron_step<GEOM, LDS>(acc, P, SP, t, s, wc, hot, wave, oe, og);
@@ -1038,6 +1059,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     }
   }
   if constexpr (HOT) hot_flush(acc, P, hot, wave);
+  if (lane_id() == 0) atomicAdd(&acc.cnt[kCntLaneSlots], 64u * trips);
   flush_eprim(acc, eprim);
   close_block(oe);
   close_block(og);
