@@ -242,17 +242,10 @@ __device__ __forceinline__ int pick_energy_bin(R lpe, R logMinE, R invLogDelta, 
   return (r1 < frac) ? ie + 1 : ie;
 }
 
-// SimLinAliasData::Sample (SimLinAliasData.cc:25-42)
+// inversion of the linear pdf within the chosen alias bin [a, b]
+// (SimLinAliasData.cc:32-41)
 template <typename R>
-__device__ __forceinline__ R alias_sample(const AliasEntry<R>* tab, int ns, uint32_t k2, uint32_t k3, int* abin) {
-  const uint32_t rest = k2 * (uint32_t)(ns - 1);
-  uint32_t i = rest >> 24;
-  const uint32_t fr = rest & 0xFFFFFFu;
-  const AliasEntry<R> ei = tab[i];
-  if (fr >= ei.thr) i = ei.alias;
-  if (abin) *abin = (int)i;
-  const AliasEntry<R> a = tab[i];
-  const AliasEntry<R> b = tab[i + 1];
+__device__ __forceinline__ R alias_invert(const AliasEntry<R>& a, const AliasEntry<R>& b, uint32_t k3) {
   const R r2 = (R)k3 * (R)(1.0 / 16777216.0);
   const R xv = a.x;
   const R xd = b.x - xv;
@@ -267,6 +260,18 @@ __device__ __forceinline__ R alias_sample(const AliasEntry<R>* tab, int ns, uint
     return xv + r2 * xd * ((R)1.0 - (R)0.5 * dum * (r2 - (R)1.0) * ((R)1.0 + dum * r2));
   }
   return xv + xd * rsqrt_(r2);
+}
+
+// SimLinAliasData::Sample (SimLinAliasData.cc:25-42)
+template <typename R>
+__device__ __forceinline__ R alias_sample(const AliasEntry<R>* tab, int ns, uint32_t k2, uint32_t k3, int* abin) {
+  const uint32_t rest = k2 * (uint32_t)(ns - 1);
+  uint32_t i = rest >> 24;
+  const uint32_t fr = rest & 0xFFFFFFu;
+  const AliasEntry<R> ei = tab[i];
+  if (fr >= ei.thr) i = ei.alias;
+  if (abin) *abin = (int)i;
+  return alias_invert(tab[i], tab[i + 1], k3);
 }
 
 // exp(xi * (ln E - ln Emin)) of the SB and Moller energy transfers in FP32:
@@ -318,24 +323,19 @@ __device__ __forceinline__ R sample_kn(const AliasSet<R>& s, R e, uint32_t k1, u
   return rexp(rlog((R)1. + (R)2. * kappa) * (xi - (R)1.));
 }
 
-// Rational inverse of SimGSTables::SampleAngularDeflection (SimGSTables.cc:
-// 96-119) for the picked energy table `ie` (and lower grid index `ielow` of
-// the transform parameter). Returns 1-cos(theta) (exact form 2aU/(1-U+a)).
+// the arithmetic of gs_rational on loaded table values: cumulative-bin
+// entries g0, g1 and the transform parameters and grid energies at ielow and
+// ielow + 1. Returns 1-cos(theta) (exact form 2aU/(1-U+a)).
 template <typename R>
-__device__ __forceinline__ R gs_rational(const GSSet<R>& s, int ie, int ielow, R e, uint32_t k2, int* cbin) {
-  // cumulative bin (int)(r2/deltaCum) == floor(r2*(ns-1)) exactly on the lattice
-  const uint32_t rest = k2 * (uint32_t)(s.ns - 1);
+__device__ __forceinline__ R gs_invert(const GSEntry<R>& g0, const GSEntry<R>& g1, R a0, R a1, R e0, R e1, R deltaCum,
+                                       int ns, uint32_t k2, R e) {
+  const uint32_t rest = k2 * (uint32_t)(ns - 1);
   const int il = (int)(rest >> 24);
-  if (cbin) *cbin = il;
-  const GSEntry<R> g0 = s.e[ie * s.ns + il];
-  const GSEntry<R> g1 = s.e[ie * s.ns + il + 1];
-  const R a0 = s.transf[ielow], a1 = s.transf[ielow + 1];
-  const R e0 = s.egrid[ielow], e1 = s.egrid[ielow + 1];
   if (sizeof(R) == 8) {
     const R r2 = (R)k2 * (R)(1.0 / 16777216.0);
-    const R resid = r2 - (R)il * s.deltaCum;
-    const R dum1 = ((R)1.0 + g0.a + g0.b) * s.deltaCum * resid;
-    const R dum2 = s.deltaCum * s.deltaCum + g0.a * s.deltaCum * resid + g0.b * resid * resid;
+    const R resid = r2 - (R)il * deltaCum;
+    const R dum1 = ((R)1.0 + g0.a + g0.b) * deltaCum * resid;
+    const R dum2 = deltaCum * deltaCum + g0.a * deltaCum * resid + g0.b * resid * resid;
     const R theU = g0.u + dum1 / dum2 * (g1.u - g0.u);
     const R parTransf = (a1 - a0) / (e1 - e0) * (e - e0) + a0;
     return (R)2. * parTransf * theU / ((R)1. - theU + parTransf);
@@ -345,7 +345,7 @@ __device__ __forceinline__ R gs_rational(const GSSet<R>& s, int ie, int ielow, R
   // 1 - U, which cancels as U -> 1, and the fraction within the cumulative
   // bin is kept in FP64: in the last bins (u1 - u0 ~ 0.03, a ~ 1e-3) an
   // FP32 fraction alone moves cos(theta) by several 1e-6.
-  const double dc = (double)s.deltaCum;
+  const double dc = (double)deltaCum;
   const double resid = (double)(rest & 0xFFFFFFu) * (1.0 / 16777216.0) * dc;
   const double ga = (double)g0.a, gb = (double)g0.b;
   const double f = (1.0 + ga + gb) * dc * resid / (dc * dc + ga * dc * resid + gb * resid * resid);
@@ -353,6 +353,18 @@ __device__ __forceinline__ R gs_rational(const GSSet<R>& s, int ie, int ielow, R
   const double omU = (double)g0.v + f * ((double)g1.v - (double)g0.v);
   const double pa = ((double)a1 - (double)a0) / ((double)e1 - (double)e0) * ((double)e - (double)e0) + (double)a0;
   return (R)(2.0 * pa * U / (omU + pa));
+}
+
+// Rational inverse of SimGSTables::SampleAngularDeflection (SimGSTables.cc:
+// 96-119) for the picked energy table `ie` (and lower grid index `ielow` of
+// the transform parameter). Returns 1-cos(theta).
+template <typename R>
+__device__ __forceinline__ R gs_rational(const GSSet<R>& s, int ie, int ielow, R e, uint32_t k2, int* cbin) {
+  // cumulative bin (int)(r2/deltaCum) == floor(r2*(ns-1)) exactly on the lattice
+  const int il = (int)((k2 * (uint32_t)(s.ns - 1)) >> 24);
+  if (cbin) *cbin = il;
+  return gs_invert(s.e[ie * s.ns + il], s.e[ie * s.ns + il + 1], s.transf[ielow], s.transf[ielow + 1],
+                   s.egrid[ielow], s.egrid[ielow + 1], s.deltaCum, s.ns, k2, e);
 }
 
 // SimGSTables::SampleAngularDeflection (SimGSTables.cc:84-119). Returns

@@ -919,8 +919,42 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
     int ielow = 0;
     const int ie = pick_energy_bin(lpe, logMinE, invLogDelta, logMinE_d, invLogDelta_d, es, k1, &ielow);
     R cost, q, esec = (R)0;
+    // FP32 tables: the first loads of both samplers are issued together for
+    // every interacting lane -- the two GS entries of the cumulative bin (MSC
+    // hinge) or the alias entry of the bin and its successor, which are the
+    // pdf's end points when the bin is kept (SB / Moller) -- so one memory
+    // round trip serves the wave; a lane whose bin is aliased loads its two
+    // entries after the test, as alias_sample does. Same values, same
+    // arithmetic as gs_rational / alias_sample.
+    AliasEntry<R> aa, ab;
+    GSEntry<R> g0, g1;
+    R ga0 = (R)0, ga1 = (R)0, ge0 = (R)0, ge1 = (R)0;
+    const AliasEntry<R>* tab = brem ? T.sb.e + (size_t)(imat * T.sb.ne + ie) * T.sb.ns : T.mo.e + (size_t)ie * T.mo.ns;
+    if constexpr (sizeof(R) == 4) {
+      const int nsel = hinge ? T.gs.ns : (brem ? T.sb.ns : T.mo.ns);
+      const uint32_t rest = k2 * (uint32_t)(nsel - 1);
+      const int i0 = (int)(rest >> 24);
+      const uint4* src = hinge ? (const uint4*)(T.gs.e + ie * T.gs.ns + i0) : (const uint4*)(tab + i0);
+      const uint4 w0 = src[0], w1 = src[1];
+      if (hinge) {
+        ga0 = T.gs.transf[ielow]; ga1 = T.gs.transf[ielow + 1];
+        ge0 = T.gs.egrid[ielow]; ge1 = T.gs.egrid[ielow + 1];
+        g0 = *reinterpret_cast<const GSEntry<R>*>(&w0);
+        g1 = *reinterpret_cast<const GSEntry<R>*>(&w1);
+      } else {
+        aa = *reinterpret_cast<const AliasEntry<R>*>(&w0);
+        ab = *reinterpret_cast<const AliasEntry<R>*>(&w1);
+        if ((rest & 0xFFFFFFu) >= aa.thr) {
+          const uint32_t ia = aa.alias;
+          aa = tab[ia];
+          ab = tab[ia + 1];
+        }
+      }
+    }
     if (hinge) {
-      const R omu = gs_rational(T.gs, ie, ielow, es, k2, (int*)nullptr);
+      R omu;
+      if constexpr (sizeof(R) == 4) omu = gs_invert(g0, g1, ga0, ga1, ge0, ge1, T.gs.deltaCum, T.gs.ns, k2, es);
+      else omu = gs_rational(T.gs, ie, ielow, es, k2, (int*)nullptr);
       if (sizeof(R) == 8) {
         cost = rmax((R)-1.0, rmin((R)1.0, (R)1. - omu));
         q = ((R)1.0 - cost) * ((R)1.0 + cost);
@@ -931,10 +965,11 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
       }
     } else {
       // SimSBTables.cc:93-114 (photon energy) or SimMollerTables.cc:69-89
-      const AliasEntry<R>* tab = brem ? T.sb.e + (size_t)(imat * T.sb.ne + ie) * T.sb.ns : T.mo.e + (size_t)ie * T.mo.ns;
       const int ns = brem ? T.sb.ns : T.mo.ns;
       const R minE = brem ? T.sb.minE : T.mo.minE;
-      const R xi = alias_sample(tab, ns, k2, k3, (int*)nullptr);
+      R xi;
+      if constexpr (sizeof(R) == 4) xi = alias_invert(aa, ab, k3);
+      else xi = alias_sample(tab, ns, k2, k3, (int*)nullptr);
       if (sizeof(R) == 8) {
         const R ex = rexp(brem ? -xi * (logMinE - lpe) : xi * (lpe - logMinE));
         esec = brem ? minE / es * ex * es : ex * (R)0.5 * minE;
