@@ -262,16 +262,23 @@ __device__ __forceinline__ R alias_invert(const AliasEntry<R>& a, const AliasEnt
   return xv + xd * rsqrt_(r2);
 }
 
-// SimLinAliasData::Sample (SimLinAliasData.cc:25-42)
+// SimLinAliasData::Sample (SimLinAliasData.cc:25-42). The bin's entry and
+// its successor are loaded together: when the bin is kept they are the pdf's
+// end points, so only an aliased bin costs a second round trip.
 template <typename R>
 __device__ __forceinline__ R alias_sample(const AliasEntry<R>* tab, int ns, uint32_t k2, uint32_t k3, int* abin) {
   const uint32_t rest = k2 * (uint32_t)(ns - 1);
   uint32_t i = rest >> 24;
   const uint32_t fr = rest & 0xFFFFFFu;
-  const AliasEntry<R> ei = tab[i];
-  if (fr >= ei.thr) i = ei.alias;
+  AliasEntry<R> a = tab[i];
+  AliasEntry<R> b = tab[i + 1];
+  if (fr >= a.thr) {
+    i = a.alias;
+    a = tab[i];
+    b = tab[i + 1];
+  }
   if (abin) *abin = (int)i;
-  return alias_invert(tab[i], tab[i + 1], k3);
+  return alias_invert(a, b, k3);
 }
 
 // exp(xi * (ln E - ln Emin)) of the SB and Moller energy transfers in FP32:
