@@ -195,7 +195,7 @@ def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=40
                       % (cores, n_each, max(secs), wall, n_each / (sum(secs) / cores))}
 
 
-def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
+def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     """Configuration 3: 256x256x128 heterogeneous phantom at 2 mm, 12 MeV e-
     10x10 cm2 field, 1e8 histories, FP32 transport, FP64 tally, 100 batches."""
     import numpy as np
@@ -204,8 +204,12 @@ def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
     ph = dpm_amd.Phantom.voxel(t, mat, rho, 2.0)
     n, nb = 100_000_000, 100
     kw = dict(particle=dpm_amd.ELECTRON, energy=12.0, field=100.0, seed=3, nbatch=nb)
-    sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0, nbatch=nb)
-    sched["chunk"] = 0
+    if fixed is not None:
+        sched = dict(fixed)
+    else:
+        sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0,
+                                 nbatch=nb)
+        sched["chunk"] = 0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     dpm_amd.simulate(t, ph, 1_000_000, schedule=sched, out=out, fetch=False, **kw)   # warm-up
     t0 = time.perf_counter()
@@ -230,7 +234,7 @@ def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
     return r
 
 
-def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
+def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     """Configuration 5: 512x512x256 heterogeneous phantom at 1 mm, 36 gantry
     photon fields (10x10 cm2, triangular 0.1/2/18 MeV, uneven counts), 1e9
     histories in one run, FP32 transport, FP64 tally, 100 batches."""
@@ -242,9 +246,12 @@ def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True):
     n = sum(f[3] for f in plan)
     kw = dict(particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0, field=100.0, seed=5)
     # schedule tuned on one field of the plan (the 45-degree beam, whose rays cross the most voxels)
-    sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
-                             field=100.0, angle=45.0, nbatch=100)
-    sched["chunk"] = 0
+    if fixed is not None:
+        sched = dict(fixed)
+    else:
+        sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
+                                 field=100.0, angle=45.0, nbatch=100)
+        sched["chunk"] = 0
     dpm_amd.simulate_fields(t, ph, phantoms.config5_fields(1e7), fetch=False, schedule=sched, **kw)   # warm-up
     t0 = time.perf_counter()
     _, _, st = dpm_amd.simulate_fields(t, ph, plan, nbatch=100, fetch=False, schedule=sched, **kw)
@@ -473,7 +480,8 @@ def main():
         tables3 = __graft_entry__.tables_dir(3)
         for key, leg in (("config3", config3_leg), ("config5", config5_leg)):
             try:
-                result[key] = leg(dpm_amd, gamma, phantoms, tables3, cpu=not args.no_cpu)
+                result[key] = leg(dpm_amd, gamma, phantoms, tables3, cpu=not args.no_cpu,
+                                  fixed=sched if fixed else None)
             except Exception as e:   # a failing extra leg is reported, the headline line still prints
                 result[key] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(result), flush=True)
