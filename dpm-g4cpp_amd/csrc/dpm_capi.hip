@@ -750,8 +750,11 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.lds_tables = (sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO &&
                   3 * T.nmat * T.itr1.n + T.moller.n + T.k1.n <= kLdsSplineNodes &&
                   T.itr1.n == T.dedx.n && T.itr1.n == T.brem.n) ? 1 : 0;
-  P.lds_photon = (sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO && T.lin_shared_grid &&
-                  (2 + 3 * T.nmat) * T.ptot.n <= kLdsLinearFloats) ? 1 : 0;
+  {
+    const int need = (2 + 3 * T.nmat) * T.ptot.n;
+    const bool ok = sizeof(R) == 4 && opt->table_memory == DPM_TABLES_AUTO && T.lin_shared_grid;
+    P.lds_photon = !ok ? 0 : (need <= kLdsLinearFloats ? 1 : (need <= kLdsLinearFloatsLarge ? 2 : 0));
+  }
 
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
@@ -907,7 +910,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
     st->kernel_calls[0] = 0; st->kernel_calls[1] = c_ph; st->kernel_calls[2] = c_el; st->kernel_calls[3] = c_fold;
     st->graphs = n_graphs;
-    st->lds_tables = P.lds_tables | (P.lds_photon << 1);
+    st->lds_tables = P.lds_tables | ((P.lds_photon != 0) << 1);
   }
   return DPM_OK;
 }

@@ -41,8 +41,12 @@ constexpr int kLdsSplineNodes = 1536;
 // capacity of the LDS copy of the photon kernel's linear tables (floats): the
 // shared energy grid, the maximum IMFP and 3 IMFPs per material, n nodes each:
 // (2 + 3 * nmat) * n = 5120 for one material of 1024 nodes, 8192 for two;
-// 32 KB, so that four blocks of the photon kernel fit in a CU's 160 KB
+// 32 KB, so that four blocks of the photon kernel fit in a CU's 160 KB. The
+// large copy (48 KB, three blocks per CU) takes three materials (11264): the
+// lookups it saves outweigh the lost block on heterogeneous phantoms, whose
+// Woodcock tracking makes many null steps.
 constexpr int kLdsLinearFloats = 8192;
+constexpr int kLdsLinearFloatsLarge = 12288;
 
 template <typename R>
 struct KParams {
@@ -62,7 +66,7 @@ struct KParams {
   int refill_min;           // idle lanes that trigger a wave refill (1..64)
   int tally_mode;           // voxel tally: 0 per-lane write-combining, 1 one atomic per deposit
   int lds_tables;           // FP32 electron kernel: spline tables copied into LDS (they fit)
-  int lds_photon;           // FP32 photon kernel: linear tables copied into LDS (they fit)
+  int lds_photon;           // FP32 photon kernel: linear tables in LDS: 0 no, 1 small copy, 2 large copy
 };
 
 // Primaries of one particle kind, enumerated in list order: entry i owns

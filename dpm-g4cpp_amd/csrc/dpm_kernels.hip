@@ -489,11 +489,15 @@ __device__ __forceinline__ void flush_eprim(BlockAcc& a, double eprim) {
 // the per-material IMFPs share one energy grid): the grid, then the maximum
 // IMFP, then the total, Compton and pair IMFPs of every material. Same
 // arithmetic in both forms.
+template <int N>
 struct LdsLinear {
-  float v[kLdsLinearFloats];
+  float v[N];
 };
-template <bool LDS> struct LdsLinOf { using type = NoLds; };
-template <> struct LdsLinOf<true> { using type = LdsLinear; };
+// LT: 0 tables in HBM, 1 the small copy (kLdsLinearFloats: four blocks of
+// the kernel fit in a CU), 2 the large one (kLdsLinearFloatsLarge: three)
+template <int LT> struct LdsLinOf { using type = NoLds; };
+template <> struct LdsLinOf<1> { using type = LdsLinear<kLdsLinearFloats>; };
+template <> struct LdsLinOf<2> { using type = LdsLinear<kLdsLinearFloatsLarge>; };
 
 template <bool LDS>
 struct PTables {
@@ -520,7 +524,7 @@ struct PTables {
 };
 
 // copy of the photon tables into the block's LDS (all threads)
-__device__ __forceinline__ void load_lds_linear(const Tables<R>& T, LdsLinear& L) {
+__device__ __forceinline__ void load_lds_linear(const Tables<R>& T, float* L) {
   const int n = T.gmax.n, nm = T.nmat * n;
   const int total = 2 * n + 3 * nm;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -530,7 +534,7 @@ __device__ __forceinline__ void load_lds_linear(const Tables<R>& T, LdsLinear& L
     else if (i < 2 * n + nm) v = (float)T.ptot.xy[2 * (i - 2 * n) + 1];
     else if (i < 2 * n + 2 * nm) v = (float)T.pcomp.xy[2 * (i - 2 * n - nm) + 1];
     else v = (float)T.ppair.xy[2 * (i - 2 * n - 2 * nm) + 1];
-    L.v[i] = v;
+    L[i] = v;
   }
 }
 
@@ -675,7 +679,7 @@ __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, i
   }
 }
 
-template <int GEOM, bool LDS>
+template <int GEOM, int LT>
 __global__ void __launch_bounds__(kBlock) DPM_PHOTON_WAVES
 k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head, QueueView<R> qe,
           QueueView<R> qg) {
@@ -687,8 +691,9 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   if ((int64_t)blockIdx.x * kBlock >= (int64_t)n_all) return;
   __shared__ typename AccOf<GEOM>::type acc;
   __shared__ int wout[kBlock / 64][2][2];
-  __shared__ typename LdsLinOf<LDS>::type lin;
-  if constexpr (LDS) load_lds_linear(P.T, lin);   // made visible by acc_init's barrier
+  constexpr bool LDS = LT != 0;
+  __shared__ typename LdsLinOf<LT>::type lin;
+  if constexpr (LDS) load_lds_linear(P.T, lin.v);   // made visible by acc_init's barrier
   PTables<LDS> PT{P.T, nullptr};
   if constexpr (LDS) PT.L = lin.v;
   const int wave = threadIdx.x >> 6;
@@ -1212,19 +1217,25 @@ hipError_t DPM_CAT(launch_photons, DPM_SFX)(const KParams<R>& P, const SourcePar
                                             int blocks, const QueueView<R>& in, int* head, const QueueView<R>& qe,
                                             const QueueView<R>& qg, hipStream_t st) {
 #if DPM_PREC == 32
-  if (P.lds_photon) {
+  if (P.lds_photon == 1) {
     if (geom == GEOM_SLAB)
-      hipLaunchKernelGGL((k_photons<GEOM_SLAB, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+      hipLaunchKernelGGL((k_photons<GEOM_SLAB, 1>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
     else
-      hipLaunchKernelGGL((k_photons<GEOM_VOXEL, true>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe,
-                         qg);
+      hipLaunchKernelGGL((k_photons<GEOM_VOXEL, 1>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    return hipGetLastError();
+  }
+  if (P.lds_photon == 2) {
+    if (geom == GEOM_SLAB)
+      hipLaunchKernelGGL((k_photons<GEOM_SLAB, 2>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    else
+      hipLaunchKernelGGL((k_photons<GEOM_VOXEL, 2>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
     return hipGetLastError();
   }
 #endif
   if (geom == GEOM_SLAB)
-    hipLaunchKernelGGL((k_photons<GEOM_SLAB, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_photons<GEOM_SLAB, 0>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   else
-    hipLaunchKernelGGL((k_photons<GEOM_VOXEL, false>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
+    hipLaunchKernelGGL((k_photons<GEOM_VOXEL, 0>), dim3(blocks), dim3(kBlock), 0, st, P, S, pr, in, head, qe, qg);
   return hipGetLastError();
 }
 

@@ -169,7 +169,7 @@ typedef struct dpm_run_options {
   int table_memory;                /* DPM_TABLES_AUTO (0): FP32 runs copy the small tables into each
                                       block's LDS when they fit -- the electron spline tables
                                       (3*nmat+2 tables of 128 nodes: up to 3 materials) and the photon
-                                      IMFP tables (2+3*nmat arrays of 1024 values: up to 2 materials);
+                                      IMFP tables (2+3*nmat arrays of 1024 values: up to 3 materials);
                                       DPM_TABLES_GLOBAL (1): read them from device memory. Both give
                                       identical results. */
 } dpm_run_options;

@@ -365,8 +365,8 @@ def test_lds_tables_match_device_tables(gpu_tables0, gpu_tables3, hetero, partic
     # FP32 electron splines and photon IMFP tables read from the block's LDS
     # copy or from the device pack: the same arithmetic on the same values, so
     # every history scores the same energy (up to the order of its FP64 tally
-    # additions). Water alone: both kernels use LDS; three materials: the
-    # photon tables (2 + 3 * 3 arrays of 1024) do not fit and stay in HBM.
+    # additions). Both kernels use LDS tables for water alone (the photon
+    # kernel's small copy) and for three materials (its large copy).
     if hetero:
         t = gpu_tables3
         mat, rho = small_heterogeneous()
@@ -377,7 +377,7 @@ def test_lds_tables_match_device_tables(gpu_tables0, gpu_tables3, hetero, partic
     kw = dict(particle=particle, energy=12.0, field=20.0, seed=23, per_history=True, nbatch=4)
     d_lds, s_lds, st_lds, h_lds = dpm_amd.simulate(t, ph, 3000, table_memory=0, **kw)
     d_dev, s_dev, st_dev, h_dev = dpm_amd.simulate(t, ph, 3000, table_memory=1, **kw)
-    assert st_lds["lds_tables"] == (1 if hetero else 3) and st_dev["lds_tables"] == 0
+    assert st_lds["lds_tables"] == 3 and st_dev["lds_tables"] == 0
     np.testing.assert_allclose(h_lds, h_dev, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(d_lds, d_dev, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(s_lds, s_dev, rtol=1e-12, atol=1e-12)
