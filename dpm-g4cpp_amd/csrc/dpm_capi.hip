@@ -278,8 +278,9 @@ struct Workspace {
       return DPM_OK;
     }
     release();
-    const size_t per = (size_t)want * (7 * sizeof(R) + 4 * sizeof(uint32_t));
-    const size_t total = 4 * (per + 11 * 256);
+    want = (want + kQLanes - 1) / kQLanes * kQLanes;
+    const size_t per = (size_t)(want / kQLanes) * kQBlockBytes<R>;
+    const size_t total = 4 * ((per + 255) & ~(size_t)255);
     if (hipMalloc(&buf, total) != hipSuccess) {
       buf = nullptr;
       return set_error(DPM_E_NOMEM, "cannot allocate particle queues of " + std::to_string(total >> 20) + " MiB");
@@ -294,11 +295,7 @@ struct Workspace {
     };
     for (int k = 0; k < 4; ++k) {
       QueueView<R>& v = q[k];
-      v.x = (R*)take(sizeof(R) * want); v.y = (R*)take(sizeof(R) * want); v.z = (R*)take(sizeof(R) * want);
-      v.u = (R*)take(sizeof(R) * want); v.v = (R*)take(sizeof(R) * want); v.w = (R*)take(sizeof(R) * want);
-      v.e = (R*)take(sizeof(R) * want);
-      v.hist = (uint32_t*)take(4 * (size_t)want); v.slo = (uint32_t*)take(4 * (size_t)want);
-      v.shi = (uint32_t*)take(4 * (size_t)want); v.tt = (uint32_t*)take(4 * (size_t)want);
+      v.base = take(per);
       v.cap = want;
       v.count = ctl + k;
     }

@@ -26,14 +26,23 @@ struct StatsDev {
   int pad;
 };
 
-// SoA particle queue; tt packs type (2 bits) | draws already used (6) | tag (24)
+// Particle queue in blocks of 64 records (AoSoA): block b holds slots
+// [64b, 64b + 64) as 7 runs of 64 transport-precision values (x y z u v w E)
+// followed by 4 runs of 64 u32 (history, serial lo, serial hi, tt). A wave's
+// 64 consecutive slots read or write one contiguous run per field, every
+// field of a slot sits at a fixed offset from its first one (an instruction
+// immediate: at most 2.75 KB in FP32), and a queue is one base pointer.
+// tt packs type (2 bits) | draws already used (6) | tag (24).
+constexpr int kQLanes = 64;
 template <typename R>
 struct QueueView {
-  R *x, *y, *z, *u, *v, *w, *e;
-  uint32_t *hist, *slo, *shi, *tt;
+  unsigned char* base;      // cap / 64 blocks of kQBlockBytes<R>
   int* count;
-  int cap;
+  int cap;                  // multiple of 64
 };
+template <typename R> constexpr int kQRealRun = kQLanes * (int)sizeof(R);
+template <typename R> constexpr int kQU32 = 7 * kQRealRun<R>;            // offset of the u32 runs in a block
+template <typename R> constexpr int kQBlockBytes = kQU32<R> + 4 * kQLanes * 4;
 
 // capacity of the LDS copy of the electron kernel's spline tables (nodes;
 // 20 B each): 3 * nmat * n + 2 * n = 1408 for 3 materials of 128 nodes

@@ -317,23 +317,39 @@ __device__ __forceinline__ uint32_t make_tt(int type, uint32_t ndraw, uint32_t t
   return (uint32_t)type | (ndraw << 2) | (tag << 8);
 }
 
+// a slot's record: field f of the 7 real fields at r + f * kQRealRun, u32
+// field g at u + g * 256 (the same 64-bit address plus an immediate in FP32)
+struct QSlot {
+  unsigned char* r;
+  unsigned char* u;
+};
+__device__ __forceinline__ QSlot qslot(const QueueView<R>& q, int slot) {
+  unsigned char* blk = q.base + (uint64_t)(uint32_t)(slot >> 6) * (uint32_t)kQBlockBytes<R>;
+  const uint32_t l = (uint32_t)slot & 63u;
+  return QSlot{blk + l * sizeof(R), blk + kQU32<R> + l * 4u};
+}
+__device__ __forceinline__ R& qreal(const QSlot& s, int f) { return *(R*)(s.r + f * kQRealRun<R>); }
+__device__ __forceinline__ uint32_t& qu32(const QSlot& s, int g) { return *(uint32_t*)(s.u + g * (kQLanes * 4)); }
+enum { kQHist = 0, kQSlo = 1, kQShi = 2, kQTT = 3 };
+
 __device__ __forceinline__ void push(const OutQ<QueueView<R>>& o, int* overflow, const R pos[3], R u, R v, R w, R e,
                                      uint32_t hist, uint64_t serial, int type, uint32_t tag) {
   const QueueView<R>& q = o.q;
   const int slot = wave_slot(q.count, o.st);
   if (slot >= q.cap) { atomicAdd(overflow, 1); return; }
-  q.x[slot] = pos[0]; q.y[slot] = pos[1]; q.z[slot] = pos[2];
-  q.u[slot] = u; q.v[slot] = v; q.w[slot] = w; q.e[slot] = e;
-  q.hist[slot] = hist;
-  q.slo[slot] = (uint32_t)serial;
-  q.shi[slot] = (uint32_t)(serial >> 32);
-  q.tt[slot] = make_tt(type, 0u, tag);
+  const QSlot s = qslot(q, slot);
+  qreal(s, 0) = pos[0]; qreal(s, 1) = pos[1]; qreal(s, 2) = pos[2];
+  qreal(s, 3) = u; qreal(s, 4) = v; qreal(s, 5) = w; qreal(s, 6) = e;
+  qu32(s, kQHist) = hist;
+  qu32(s, kQSlo) = (uint32_t)serial;
+  qu32(s, kQShi) = (uint32_t)(serial >> 32);
+  qu32(s, kQTT) = make_tt(type, 0u, tag);
 }
 
 __device__ __forceinline__ void close_block(const OutQ<QueueView<R>>& o) {
   const int base = o.st[0], used = o.st[1];
   const int l = lane_id();
-  if (used < kBlk && l >= used && l < kBlk && base + l < o.q.cap) o.q.tt[base + l] = kEmptySlot;
+  if (used < kBlk && l >= used && l < kBlk && base + l < o.q.cap) qu32(qslot(o.q, base + l), kQTT) = kEmptySlot;
 }
 
 struct Track {
@@ -348,17 +364,18 @@ struct Track {
 };
 
 __device__ __forceinline__ void load_track(const QueueView<R>& q, int i, const KParams<R>& P, Track& t) {
-  t.pos[0] = q.x[i]; t.pos[1] = q.y[i]; t.pos[2] = q.z[i];
-  t.dir[0] = q.u[i]; t.dir[1] = q.v[i]; t.dir[2] = q.w[i];
-  t.E = q.e[i];
-  const uint32_t tt = q.tt[i];
+  const QSlot s = qslot(q, i);
+  t.pos[0] = qreal(s, 0); t.pos[1] = qreal(s, 1); t.pos[2] = qreal(s, 2);
+  t.dir[0] = qreal(s, 3); t.dir[1] = qreal(s, 4); t.dir[2] = qreal(s, 5);
+  t.E = qreal(s, 6);
+  const uint32_t tt = qu32(s, kQTT);
   t.type = (int)(tt & 3u);
   t.tag = tt >> 8;
   t.iter = 0;
   t.rng.k0 = P.k0; t.rng.k1 = P.k1;
-  t.rng.hist = q.hist[i];
-  t.rng.slo = q.slo[i];
-  t.rng.shi = q.shi[i];
+  t.rng.hist = qu32(s, kQHist);
+  t.rng.slo = qu32(s, kQSlo);
+  t.rng.shi = qu32(s, kQShi);
   t.rng.ndraw = (tt >> 2) & 63u;
   t.rng.nchild = 0;
   // nothing buffered: the first top-up (or draw) computes block ndraw>>2, so a
