@@ -274,7 +274,7 @@ struct Workspace {
   }
   int ensure(int want, int* ctl) {
     if (cap >= want && buf) {
-      for (int k = 0; k < 4; ++k) q[k].count = ctl + k;
+      for (int k = 0; k < 4; ++k) q[k].count = (unsigned long long*)ctl + k;
       return DPM_OK;
     }
     release();
@@ -297,7 +297,7 @@ struct Workspace {
       QueueView<R>& v = q[k];
       v.base = take(per);
       v.cap = want;
-      v.count = ctl + k;
+      v.count = (unsigned long long*)ctl + k;
     }
     return DPM_OK;
   }
@@ -311,10 +311,11 @@ constexpr int kRoundsPerSync = 8;
 // clear the next round's queue counts and both consumption heads
 __global__ void k_round_prep(int* ctl, int nxt) {
   if (threadIdx.x == 0) {
-    ctl[nxt] = 0;
-    ctl[2 + nxt] = 0;
-    ctl[4] = 0;
-    ctl[6] = 0;
+    unsigned long long* cq = (unsigned long long*)ctl;
+    cq[nxt] = 0;
+    cq[2 + nxt] = 0;
+    ctl[kCtlHeadG] = 0;
+    ctl[kCtlHeadE] = 0;
   }
 }
 
@@ -502,7 +503,7 @@ struct dpm_tables {
   void* dev_d = nullptr;
   Workspace<float> wsf;
   Workspace<double> wsd;
-  int* ctl = nullptr;        // [0..3] queue counts, [4] head, [5] overflow
+  int* ctl = nullptr;        // [0..7] packed queue counts, kCtlHeadG/kCtlHeadE heads, kCtlOverflow
   StatsDev* stats = nullptr;
   hipEvent_t ev[10 + 4 * kRoundsPerSync] = {};  // [10 + 4k ..]: kernel bounds of round k of a sync block
   // tally buffers reused across runs
@@ -740,7 +741,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.hist_edep = d_hist_edep;
   P.hist_base = opt->hist_begin;
   P.stats = t->stats;
-  P.overflow = t->ctl + 5;
+  P.overflow = t->ctl + kCtlOverflow;
   P.refill_min = std::min(64, std::max(1, opt->refill_min));
   P.tally_mode = (opt->tally_mode == 1 || opt->tally_mode == 2) ? opt->tally_mode : 0;
   // FP32: the electron kernel's spline tables go to LDS when they fit
@@ -820,7 +821,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
     const PrimRange r_ph{std::min(a, n_ph), std::min(b, n_ph), bfirst};
     const PrimRange r_el{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst};
     const PrimRange none{0, 0, bfirst};
-    HIP_TRY(hipMemsetAsync(t->ctl, 0, 8 * sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(t->ctl, 0, kCtlInts * sizeof(int), s));
     // rps rounds of photons then electrons; the first generates
     // this chunk's primaries. Round r reads queues `cur` and appends photons
     // to `nxt`; electrons of both kernels go to the electron queue the
@@ -833,12 +834,12 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
         if (timed) (void)hipEventRecord(t->ev[10 + 4 * k], s);
         P.kind = 0;
         hipError_t e = Launch<R>::photons(P, dS, with_primaries && k == 0 ? r_ph : none, p->geom, blocks,
-                                          ws.q[0 + c], t->ctl + 4, ws.q[2 + c], ws.q[0 + nx], s);
+                                          ws.q[0 + c], t->ctl + kCtlHeadG, ws.q[2 + c], ws.q[0 + nx], s);
         if (e != hipSuccess) return e;
         if (timed) { (void)hipEventRecord(t->ev[11 + 4 * k], s); (void)hipEventRecord(t->ev[12 + 4 * k], s); }
         P.kind = 1;
         e = Launch<R>::electrons(P, dS, with_primaries && k == 0 ? r_el : none, p->geom, blocks, ws.q[2 + c],
-                                 t->ctl + 6, ws.q[2 + nx], ws.q[0 + nx], s);
+                                 t->ctl + kCtlHeadE, ws.q[2 + nx], ws.q[0 + nx], s);
         if (e != hipSuccess) return e;
         if (timed) (void)hipEventRecord(t->ev[13 + 4 * k], s);
         c = nx;
@@ -868,8 +869,8 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
       } else {
         HIP_TRY(enqueue(first, true));
       }
-      int counts[6];
-      HIP_TRY(hipMemcpyAsync(counts, t->ctl, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
+      int counts[kCtlInts];
+      HIP_TRY(hipMemcpyAsync(counts, t->ctl, kCtlInts * sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
       first = false;
       if (!use_graph)
@@ -879,11 +880,12 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
           ++c_ph; ++c_el;
         }
       rounds += rps;
-      if (counts[5] != 0)
+      if (counts[kCtlOverflow] != 0)
         return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
                                              "); use a smaller chunk");
       // an even number of rounds leaves the next inputs in queues 0 and 2
-      if (counts[0] == 0 && counts[2] == 0) break;
+      // (packed counters: ints 0-1 and 4-5)
+      if ((counts[0] | counts[1] | counts[4] | counts[5]) == 0) break;
       if (rounds > 100000) return set_error(DPM_E_DEVICE, "transport did not drain");
     }
     if (nbatch > 0) {
@@ -1058,7 +1060,7 @@ int dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out) 
     return rc;
   }
   if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&t->ctl, 16 * sizeof(int)) != hipSuccess || hipMalloc(&t->stats, sizeof(StatsDev)) != hipSuccess) {
+      hipMalloc(&t->ctl, kCtlInts * sizeof(int)) != hipSuccess || hipMalloc(&t->stats, sizeof(StatsDev)) != hipSuccess) {
     dpm_tables_destroy(t);
     return set_error(DPM_E_DEVICE, "cannot create the run stream or control block");
   }

@@ -33,13 +33,24 @@ struct StatsDev {
 // field of a slot sits at a fixed offset from its first one (an instruction
 // immediate: at most 2.75 KB in FP32), and a queue is one base pointer.
 // tt packs type (2 bits) | draws already used (6) | tag (24).
+// Electron queues are filled from both ends: Compton electrons below
+// kBackBelow MeV from slot cap - 1 downwards (the back run), every other
+// record from slot 0 upwards (the front run); consumers take the front run
+// first, so that the particles still in flight when a kernel's input runs
+// out are short-lived ones and the kernel's tail is short. One 64-bit counter holds
+// both run lengths (front: low 32 bits, back: high 32 bits), so a
+// reservation sees the other run and the two can never overlap.
 constexpr int kQLanes = 64;
 template <typename R>
 struct QueueView {
-  unsigned char* base;      // cap / 64 blocks of kQBlockBytes<R>
-  int* count;
-  int cap;                  // multiple of 64
+  unsigned char* base;          // cap / 64 blocks of kQBlockBytes<R>
+  unsigned long long* count;    // front | back << 32 (slots reserved)
+  int cap;                      // multiple of 64
 };
+constexpr float kBackBelow = 1.0f;
+// control words of a run (int index into the device array; the four packed
+// queue counters G0, G1, E0, E1 take ints 0..7)
+constexpr int kCtlHeadG = 8, kCtlOverflow = 9, kCtlHeadE = 10, kCtlInts = 16;
 template <typename R> constexpr int kQRealRun = kQLanes * (int)sizeof(R);
 template <typename R> constexpr int kQU32 = 7 * kQRealRun<R>;            // offset of the u32 runs in a block
 template <typename R> constexpr int kQBlockBytes = kQU32<R> + 4 * kQLanes * 4;

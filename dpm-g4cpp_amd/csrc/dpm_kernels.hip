@@ -70,27 +70,50 @@ constexpr uint32_t kEmptySlot = 3u;   // type code of an unfilled queue slot
 template <typename Q>
 struct OutQ {
   Q q;
-  volatile int* st;   // this wave's [base, used] in LDS
+  volatile int* st;    // this wave's [base, used] of the front run in LDS
+  volatile int* stb;   // the same for the back run (electron queues), or null
 };
 
-__device__ __forceinline__ int wave_slot(int* counter, volatile int* st) {
+// Slots for the active lanes, each in the front run (back = false) or the
+// back run (only where the queue has one: stb not null), counted from that
+// run's end of the queue; a slot >= cap when the queue is full. Both runs go
+// through one code path, and one 64-bit atomic takes new blocks for both;
+// a reservation that would make the runs meet fails.
+__device__ __forceinline__ int wave_slot(unsigned long long* counter, int cap, volatile int* st, volatile int* stb,
+                                         bool back) {
   const uint64_t act = __ballot(1);
-  const int k = __popcll(act);
-  const int rank = (int)__popcll(act & lanes_below());
+  const uint64_t bm = stb ? __ballot(back) : 0ull;
+  const uint64_t fm = act & ~bm;
+  const int kf = __popcll(fm), kb = __popcll(bm);
   const int leader = __ffsll((long long)act) - 1;
-  const int base = st[0];
-  const int used = st[1];
-  const int room = kBlk - used;
-  int slot;
-  if (k <= room) {
-    slot = base + used + rank;
-    if (lane_id() == leader) st[1] = used + k;
-  } else {
-    int nb = 0;
-    if (lane_id() == leader) nb = atomicAdd(counter, kBlk);
-    nb = __shfl(nb, leader);
-    slot = rank < room ? base + used + rank : nb + (rank - room);
-    if (lane_id() == leader) { st[0] = nb; st[1] = k - room; }
+  const int fbase = st[0], fused = st[1];
+  const int bbase = stb ? stb[0] : 0, bused = stb ? stb[1] : 0;
+  const int froom = kBlk - fused, broom = kBlk - bused;
+  const bool fnew = kf > froom, bnew = kb > broom;
+  int fnb = 0, bnb = 0;
+  if (fnew || bnew) {   // the same in every lane
+    if (lane_id() == leader) {
+      const unsigned long long inc =
+          (fnew ? (unsigned long long)kBlk : 0ull) | (bnew ? (unsigned long long)kBlk << 32 : 0ull);
+      const unsigned long long old = atomicAdd(counter, inc);
+      const uint64_t f = old & 0xffffffffull, b = old >> 32;
+      const bool ok = f + b + (fnew ? kBlk : 0) + (bnew ? kBlk : 0) <= (uint64_t)cap;
+      fnb = ok ? (int)f : cap;
+      bnb = ok ? (int)b : cap;
+    }
+    fnb = __shfl(fnb, leader);
+    bnb = __shfl(bnb, leader);
+  }
+  const int rank = (int)__popcll((back ? bm : fm) & lanes_below());
+  const int slot = back ? (rank < broom ? bbase + bused + rank : bnb + (rank - broom))
+                        : (rank < froom ? fbase + fused + rank : fnb + (rank - froom));
+  if (lane_id() == leader) {
+    if (fnew) { st[0] = fnb; st[1] = kf - froom; }
+    else st[1] = fused + kf;
+    if (stb) {
+      if (bnew) { stb[0] = bnb; stb[1] = kb - broom; }
+      else stb[1] = bused + kb;
+    }
   }
   return slot;
 }
@@ -332,12 +355,19 @@ __device__ __forceinline__ R& qreal(const QSlot& s, int f) { return *(R*)(s.r + 
 __device__ __forceinline__ uint32_t& qu32(const QSlot& s, int g) { return *(uint32_t*)(s.u + g * (kQLanes * 4)); }
 enum { kQHist = 0, kQSlo = 1, kQShi = 2, kQTT = 3 };
 
+// SPLIT: a record below kBackBelow MeV goes to the back run. Only the
+// Compton electrons of the photon kernel are split: they are most of the
+// electron kernel's input, and the extra code runs in almost every trip of
+// the photon kernel, so the rarer pair and Moller electrons keep the
+// front-only path.
+template <bool SPLIT = false>
 __device__ __forceinline__ void push(const OutQ<QueueView<R>>& o, int* overflow, const R pos[3], R u, R v, R w, R e,
                                      uint32_t hist, uint64_t serial, int type, uint32_t tag) {
   const QueueView<R>& q = o.q;
-  const int slot = wave_slot(q.count, o.st);
+  const bool back = SPLIT && e < (R)kBackBelow;
+  const int slot = wave_slot(q.count, q.cap, o.st, SPLIT ? o.stb : nullptr, back);
   if (slot >= q.cap) { atomicAdd(overflow, 1); return; }
-  const QSlot s = qslot(q, slot);
+  const QSlot s = qslot(q, back ? q.cap - 1 - slot : slot);
   qreal(s, 0) = pos[0]; qreal(s, 1) = pos[1]; qreal(s, 2) = pos[2];
   qreal(s, 3) = u; qreal(s, 4) = v; qreal(s, 5) = w; qreal(s, 6) = e;
   qu32(s, kQHist) = hist;
@@ -346,10 +376,30 @@ __device__ __forceinline__ void push(const OutQ<QueueView<R>>& o, int* overflow,
   qu32(s, kQTT) = make_tt(type, 0u, tag);
 }
 
-__device__ __forceinline__ void close_block(const OutQ<QueueView<R>>& o) {
-  const int base = o.st[0], used = o.st[1];
+__device__ __forceinline__ void close_run(const QueueView<R>& q, volatile int* st, bool back) {
+  const int base = st[0], used = st[1];
   const int l = lane_id();
-  if (used < kBlk && l >= used && l < kBlk && base + l < o.q.cap) qu32(qslot(o.q, base + l), kQTT) = kEmptySlot;
+  if (used < kBlk && l >= used && l < kBlk && base + l < q.cap)
+    qu32(qslot(q, back ? q.cap - 1 - (base + l) : base + l), kQTT) = kEmptySlot;
+}
+__device__ __forceinline__ void close_block(const OutQ<QueueView<R>>& o) {
+  close_run(o.q, o.st, false);
+  if (o.stb) close_run(o.q, o.stb, true);
+}
+
+// records in a queue: the front run, then the back run
+struct InCount {
+  int front, n;
+};
+__device__ __forceinline__ InCount in_count(const QueueView<R>& q) {
+  const unsigned long long c = *q.count;
+  const int f = (int)min(c & 0xffffffffull, (unsigned long long)q.cap);
+  const int b = (int)min(c >> 32, (unsigned long long)(q.cap - f));
+  return InCount{f, f + b};
+}
+// queue slot of input index i < n
+__device__ __forceinline__ int in_slot(const QueueView<R>& q, const InCount& c, int i) {
+  return i < c.front ? i : q.cap - 1 - (i - c.front);
 }
 
 struct Track {
@@ -603,7 +653,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, co
       R du = sint * cp, dv = sint * sp, dw = elCost;
       rotate_to_lab(du, dv, dw, t.dir[0], t.dir[1], t.dir[2]);
       count(a, kCntSecondaries);
-      push(qe, P.overflow, t.pos, du, dv, dw, elEner, t.rng.hist, t.rng.next_child(), 1, t.tag);
+      push<true>(qe, P.overflow, t.pos, du, dv, dw, elEner, t.rng.hist, t.rng.next_child(), 1, t.tag);
     }
     t.E = phEner;
     if (t.E < T.gcut) {
@@ -703,11 +753,12 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   // a launch with less work than the grid holds lanes keeps only the blocks
   // it needs (the others would find the input exhausted); an empty drain
   // round costs one load of the queue count per block
-  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const InCount cin = in_count(in);
+  const int n_in = cin.n;
   const int n_all = n_in + (int)(pr.end - pr.beg);
   if ((int64_t)blockIdx.x * kBlock >= (int64_t)n_all) return;
   __shared__ typename AccOf<GEOM>::type acc;
-  __shared__ int wout[kBlock / 64][2][2];
+  __shared__ int wout[kBlock / 64][3][2];
   constexpr bool LDS = LT != 0;
   __shared__ typename LdsLinOf<LT>::type lin;
   if constexpr (LDS) load_lds_linear(P.T, lin.v);   // made visible by acc_init's barrier
@@ -717,10 +768,11 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
   if (lane_id() == 0) {
     wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
     wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
+    wout[wave][2][0] = 0; wout[wave][2][1] = kBlk;
   }
   acc_init(acc, GEOM == GEOM_SLAB);
-  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
-  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
+  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0], &wout[wave][2][0]};
+  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0], nullptr};
   Track t;
   double eprim = 0.0;
   bool has = false, exhausted = false;
@@ -730,7 +782,7 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
     // compaction interval: refill once refill_min lanes are idle (or none is busy)
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
-        if (idx < n_in) load_track(in, idx, P, t);
+        if (idx < n_in) load_track(in, in_slot(in, cin, idx), P, t);
         else if (!gen_primary(P, S, S->list[0], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 0)) {
           escape(acc, t.E);
           return false;
@@ -1063,11 +1115,12 @@ __global__ void __launch_bounds__(kBlock) DPM_ELECTRON_WAVES
 k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueView<R> in, int* head,
             QueueView<R> qe, QueueView<R> qg) {
   // blocks beyond the launch's work leave at once (see k_photons)
-  const int n_in = *in.count < in.cap ? *in.count : in.cap;
+  const InCount cin = in_count(in);
+  const int n_in = cin.n;
   const int n_all = n_in + (int)(pr.end - pr.beg);
   if ((int64_t)blockIdx.x * kBlock >= (int64_t)n_all) return;
   __shared__ typename AccOf<GEOM>::type acc;
-  __shared__ int wout[kBlock / 64][2][2];
+  __shared__ int wout[kBlock / 64][3][2];
   __shared__ typename LdsOf<LDS>::type lsp;
   if constexpr (LDS) load_lds_splines(P.T, lsp);   // made visible by acc_init's barrier
   ESplines<LDS> SP{P.T, nullptr, make_int4(0, 0, 0, 0)};
@@ -1081,10 +1134,11 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
   if (lane_id() == 0) {
     wout[wave][0][0] = 0; wout[wave][0][1] = kBlk;
     wout[wave][1][0] = 0; wout[wave][1][1] = kBlk;
+    wout[wave][2][0] = 0; wout[wave][2][1] = kBlk;
   }
   acc_init(acc, GEOM == GEOM_SLAB);
-  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0]};
-  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0]};
+  const OutQ<QueueView<R>> oe{qe, &wout[wave][0][0], nullptr};   // Moller electrons: front run only
+  const OutQ<QueueView<R>> og{qg, &wout[wave][1][0], nullptr};
   Track t;
   EState s;
   WComb wc{-1, 0.0};
@@ -1095,7 +1149,7 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
   for (;;) {
     if (!exhausted && (__popcll(__ballot(!has)) >= P.refill_min || !__any(has)))
       refill(has, exhausted, rng_in, head, n_all, [&](int idx) {
-        if (idx < n_in) load_track(in, idx, P, t);
+        if (idx < n_in) load_track(in, in_slot(in, cin, idx), P, t);
         else if (!gen_primary(P, S, S->list[1], pr.batch_first, pr.beg + (uint64_t)(idx - n_in), t, eprim, 1)) {
           escape(acc, t.E);
           return false;
