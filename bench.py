@@ -207,8 +207,9 @@ def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     if fixed is not None:
         sched = dict(fixed)
     else:
+        # sample: 1e7 histories in batches of the real run's size (1e6)
         sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0,
-                                 nbatch=nb)
+                                 nbatch=10)
         sched["chunk"] = 0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     dpm_amd.simulate(t, ph, 1_000_000, schedule=sched, out=out, fetch=False, **kw)   # warm-up
@@ -245,12 +246,15 @@ def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     plan = phantoms.config5_fields(1e9)
     n = sum(f[3] for f in plan)
     kw = dict(particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0, field=100.0, seed=5)
-    # schedule tuned on one field of the plan (the 45-degree beam, whose rays cross the most voxels)
+    # schedule tuned on one field of the plan (the 45-degree beam, whose rays
+    # cross the most voxels): 1.6e7 histories in two batches of about the
+    # real run's batch size (1e7), so that, as in the run, every chunk holds
+    # one batch and one batch row
     if fixed is not None:
         sched = dict(fixed)
     else:
-        sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
-                                 field=100.0, angle=45.0, nbatch=100)
+        sched = dpm_amd.autotune(t, ph, nsample=16_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
+                                 field=100.0, angle=45.0, nbatch=2)
         sched["chunk"] = 0
     dpm_amd.simulate_fields(t, ph, phantoms.config5_fields(1e7), fetch=False, schedule=sched, **kw)   # warm-up
     t0 = time.perf_counter()
@@ -344,6 +348,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch schedule instead of the autotuner's")
     ap.add_argument("--refill-min", type=int, default=0, help="fix the refill threshold (skips the autotuner)")
     ap.add_argument("--tally-mode", type=int, default=-1, help="fix the voxel-tally mode (skips the autotuner)")
+    ap.add_argument("--pipes", type=int, default=0, help="fix the number of pipes, 1 or 2 (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="beamlet leg: skip the plain per-beamlet runs")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the configuration 3 and 5 legs (full-size heterogeneous runs)")
@@ -392,10 +397,11 @@ def main():
 
     # launch schedule: the autotuner's choice for this source (outside the
     # timed region), unless fixed on the command line
-    fixed = args.no_tune or args.waves_per_cu or args.chunk or args.refill_min or args.tally_mode >= 0
+    fixed = (args.no_tune or args.waves_per_cu or args.chunk or args.refill_min or args.tally_mode >= 0 or
+             args.pipes)
     if fixed:
         sched = {"chunk": args.chunk, "waves_per_cu": args.waves_per_cu, "refill_min": args.refill_min,
-                 "tally_mode": max(0, args.tally_mode), "source": "command line / defaults"}
+                 "tally_mode": max(0, args.tally_mode), "pipes": args.pipes, "source": "command line / defaults"}
     else:
         ns = min(nhist, 1 << 24)
         sched = dpm_amd.autotune(t, ph, nsample=ns, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field,
@@ -450,6 +456,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches,
+                     # with two pipes, launches of the two streams share the GPU: the
+                     # per-launch rate above counts each launch over its whole duration;
+                     # this one divides the kernel's bytes of a step by the step's device time
+                     "pipes": int(sched.get("pipes") or 1),
+                     "achieved_per_device_second": algorithmic_bytes(cnt) / st["seconds"] / 1e9,
                      "measured_copy_GBs": copy_gbs, "atomics": atomics},
         "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
         "device_seconds_per_step": st["seconds"],

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -322,6 +323,53 @@ __global__ void k_round_prep(int* ctl, int nxt) {
 // histories per queue fill when the caller does not choose
 constexpr int64_t kDefaultChunk = (int64_t)1 << 24;
 
+// A run's chunks go to kPipes pipes, each with its own stream, queues and
+// control words, that run side by side: a pipe's kernel tails, drain rounds
+// and folds overlap the other pipe's transport (a persistent kernel's
+// blocks leave one by one as its lanes run dry, and the other stream's
+// kernel takes their place).
+constexpr int kPipes = 2;
+constexpr int64_t kPipeChunk = (int64_t)1 << 23;
+
+// histories per chunk and batch rows per chunk (per pipe)
+struct ChunkPlan {
+  int64_t chunk;
+  int rows;
+};
+uint64_t batch_start(uint64_t b, uint64_t n, uint64_t nb);
+// A caller's chunk is kept (at most kDefaultChunk), and so is the default
+// kDefaultChunk on one pipe. On two pipes the run is cut into an even
+// number of chunks of about kPipeChunk histories at most, of whole batches
+// when there are batches, so that both pipes have work.
+ChunkPlan plan_chunks(uint64_t nhist, int nbatch, int64_t opt_chunk, int npipes) {
+  ChunkPlan c{1, 1};
+  if (nhist == 0) return c;
+  if (opt_chunk > 0 || npipes < 2) {
+    if (opt_chunk <= 0) opt_chunk = kDefaultChunk;
+    c.chunk = std::min<int64_t>(std::min<int64_t>(opt_chunk, kDefaultChunk), (int64_t)nhist);
+    if (nbatch > 0) {
+      const uint64_t per = nhist / nbatch + 1;
+      c.rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)c.chunk / per));
+    }
+    return c;
+  }
+  uint64_t nch = std::max<uint64_t>(npipes, (nhist + kPipeChunk - 1) / kPipeChunk);
+  nch = (nch + npipes - 1) / npipes * npipes;
+  if (nbatch > 0) {
+    const uint64_t bpc = std::max<uint64_t>(1, ((uint64_t)nbatch + nch - 1) / nch);
+    c.rows = (int)bpc;
+    uint64_t span = 0;   // the longest run of bpc consecutive batches
+    for (uint64_t b = 0; b < (uint64_t)nbatch; b += bpc)
+      span = std::max(span, batch_start(std::min<uint64_t>(b + bpc, nbatch), nhist, nbatch) -
+                                batch_start(b, nhist, nbatch));
+    c.chunk = (int64_t)std::min<uint64_t>(span, (uint64_t)kDefaultChunk);
+  } else {
+    c.chunk = (int64_t)((nhist + nch - 1) / nch);
+  }
+  c.chunk = std::max<int64_t>(1, c.chunk);
+  return c;
+}
+
 // grow-only device buffer kept by a tables handle across runs
 struct DevBuf {
   void* p = nullptr;
@@ -501,11 +549,16 @@ struct dpm_tables {
   Tables<double> td;
   void* dev_f = nullptr;
   void* dev_d = nullptr;
-  Workspace<float> wsf;
-  Workspace<double> wsd;
-  int* ctl = nullptr;        // [0..7] packed queue counts, kCtlHeadG/kCtlHeadE heads, kCtlOverflow
+  hipStream_t pipe_stream = nullptr;   // the second pipe's stream (the first pipe uses `stream`)
+  Workspace<float> wsf[kPipes];
+  Workspace<double> wsd[kPipes];
+  int* ctl = nullptr;        // kPipes x kCtlInts: [0..7] packed queue counts, heads, overflow
+  int* ctl_host = nullptr;   // pinned copy of ctl, read after each block of rounds
   StatsDev* stats = nullptr;
-  hipEvent_t ev[10 + 4 * kRoundsPerSync] = {};  // [10 + 4k ..]: kernel bounds of round k of a sync block
+  hipEvent_t ev[4] = {};     // run start, run end, pipe join, fold order
+  // per pipe: [4k..4k+3] kernel bounds of round k of a sync block, then
+  // block done, fold start, fold end
+  hipEvent_t pev[kPipes][4 * kRoundsPerSync + 3] = {};
   // tally buffers reused across runs
   DevBuf buf_rows, buf_steps, buf_sum, buf_sum2, buf_he;
   // leading bytes of buf_rows known to be zero (left so by the folds of a
@@ -722,9 +775,12 @@ uint64_t batch_start(uint64_t b, uint64_t n, uint64_t nb) {
 // `rows` (rows x nbins|nvox doubles). Histories are taken in chunks over the
 // concatenation [photon list | e-/e+ list]; the first round of a chunk
 // generates its primaries inside the transport kernels.
+// pipes of a run (dpm_run_options.pipes, checked by the entry points)
+int pipe_count(const dpm_run_options* opt) { return opt->pipes == 2 ? 2 : 1; }
+
 template <typename R>
 int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, const dpm_run_options* opt,
-                  Workspace<R>& ws, Tables<R>& T, double* d_rows, int nrows_per_chunk, double* d_steps,
+                  Workspace<R>* ws, Tables<R>& T, const ChunkPlan& plan, double* d_rows, double* d_steps,
                   double* d_sum, double* d_sum2, double* d_hist_edep, dpm_stats* st) {
   const hipStream_t s = t->stream;
   KParams<R> P;
@@ -741,7 +797,6 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   P.hist_edep = d_hist_edep;
   P.hist_base = opt->hist_begin;
   P.stats = t->stats;
-  P.overflow = t->ctl + kCtlOverflow;
   P.refill_min = std::min(64, std::max(1, opt->refill_min));
   P.tally_mode = (opt->tally_mode == 1 || opt->tally_mode == 2) ? opt->tally_mode : 0;
   // FP32: the electron kernel's spline tables go to LDS when they fit
@@ -763,25 +818,58 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   for (int k = 0; k < 2; ++k)
     for (const HostEntry& h : hs.list[k]) single = single && h.row < 0;
   const int nbatch = (single && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
-  int64_t chunk = opt->chunk > 0 ? opt->chunk : kDefaultChunk;
-  if (chunk > ((int64_t)1 << 24)) chunk = (int64_t)1 << 24;
-  chunk = std::min<int64_t>(chunk, (int64_t)nhist);
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(plan.chunk, kDefaultChunk));
+  const int nrows_per_chunk = std::max(1, plan.rows);
+  const int npipes = pipe_count(opt);
   const int wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
   const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
   // secondaries of a chunk, plus the unfilled tails of the per-wave append
-  // blocks (at most one block per wave, output queue and kernel launch)
-  const int64_t holes = (int64_t)2 * blocks * (256 / 64) * 64;
+  // blocks (at most one block per wave and run of an output queue -- the
+  // electron queues have two runs -- and kernel launch)
+  const int64_t holes = (int64_t)3 * blocks * (256 / 64) * 64;
   const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + holes + 65536, (int64_t)1 << 30);
-  int rc = ws.ensure(cap, t->ctl);
-  if (rc != DPM_OK) return rc;
+  for (int q = 0; q < npipes; ++q) {
+    const int rc = ws[q].ensure(cap, t->ctl + q * kCtlInts);
+    if (rc != DPM_OK) return rc;
+  }
   SourceParams* dS = nullptr;
-  rc = upload_source(t, hs, p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox, nbatch,
-                     opt->hist_begin, nhist, &dS);
+  int rc = upload_source(t, hs, p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox, nbatch,
+                         opt->hist_begin, nhist, &dS);
   if (rc != DPM_OK) return rc;
 
-  HIP_TRY(hipEventRecord(t->ev[0], s));
-  uint64_t done = 0;
-  uint64_t next_batch = 0;
+  // the chunks in history order: whole batches per chunk (at least one, as
+  // many as fit in `chunk` histories and the pipe's batch rows)
+  struct Chunk {
+    PrimRange r_ph, r_el;
+    uint32_t bfirst;
+    int nrows;
+  };
+  std::vector<Chunk> chunks;
+  {
+    uint64_t done = 0, next_batch = 0;
+    while (done < nhist) {
+      uint64_t n = std::min<uint64_t>((uint64_t)chunk, nhist - done);
+      uint32_t bfirst = 0;
+      int nrows = 1;
+      if (nbatch > 0) {
+        const uint64_t b0 = next_batch;
+        uint64_t b1 = b0 + 1;
+        while (b1 < (uint64_t)nbatch && (int)(b1 - b0) < nrows_per_chunk &&
+               batch_start(b1 + 1, nhist, nbatch) - batch_start(b0, nhist, nbatch) <= (uint64_t)chunk)
+          ++b1;
+        n = batch_start(b1, nhist, nbatch) - batch_start(b0, nhist, nbatch);
+        bfirst = (uint32_t)b0;
+        nrows = (int)(b1 - b0);
+        next_batch = b1;
+      }
+      // this chunk's primaries in each kind's index space
+      const uint64_t a = done, b = done + n;
+      chunks.push_back(Chunk{PrimRange{std::min(a, n_ph), std::min(b, n_ph), bfirst},
+                             PrimRange{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst}, bfirst, nrows});
+      done += n;
+    }
+  }
+
   int rounds = 0;
   float ms_ph = 0, ms_el = 0, ms_fold = 0;
   int c_ph = 0, c_el = 0, c_fold = 0;
@@ -790,120 +878,174 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
                                     (opt->rounds_per_sync & 1)))
     return set_error(DPM_E_INVALID, "rounds_per_sync must be even, 2.." + std::to_string(kRoundsPerSync));
   const int rps = opt->rounds_per_sync ? opt->rounds_per_sync : kRoundsPerSync;
-  hipGraphExec_t g_first = nullptr, g_rest = nullptr;
   int n_graphs = 0;
-  struct GraphGuard {
-    hipGraphExec_t* a;
-    hipGraphExec_t* b;
-    ~GraphGuard() {
-      if (*a) (void)hipGraphExecDestroy(*a);
-      if (*b) (void)hipGraphExecDestroy(*b);
+  constexpr int kDone = 4 * kRoundsPerSync, kFold0 = kDone + 1, kFold1 = kDone + 2;
+
+  // one pipe: its stream, queues, control words and the chunk it runs
+  struct PipeRun {
+    hipStream_t s = nullptr;
+    Workspace<R>* ws = nullptr;
+    int* ctl = nullptr;
+    const int* host = nullptr;
+    hipEvent_t* ev = nullptr;
+    KParams<R> P;
+    hipGraphExec_t g_first = nullptr, g_rest = nullptr;
+    size_t chunk = 0;
+    bool first = false, fold_timed = false;
+  };
+  PipeRun pr[kPipes];
+  const int64_t nb = p->geom == GEOM_SLAB ? p->nbins : p->nvox;
+  for (int q = 0; q < kPipes; ++q) {
+    PipeRun& u = pr[q];
+    u.s = q == 0 ? s : t->pipe_stream;
+    u.ws = &ws[q];
+    u.ctl = t->ctl + q * kCtlInts;
+    u.host = t->ctl_host + q * kCtlInts;
+    u.ev = t->pev[q];
+    u.P = P;
+    u.P.overflow = u.ctl + kCtlOverflow;
+    // batch rows: each pipe its own (folded after each of its chunks);
+    // otherwise both pipes add into the same rows
+    if (nbatch > 0) u.P.tally = d_rows + (size_t)q * nrows_per_chunk * nb;
+  }
+  // on every exit: let both pipes finish, then free their graphs
+  struct Guard {
+    PipeRun* pr;
+    ~Guard() {
+      for (int q = 0; q < kPipes; ++q) (void)hipStreamSynchronize(pr[q].s);
+      for (int q = 0; q < kPipes; ++q) {
+        if (pr[q].g_first) (void)hipGraphExecDestroy(pr[q].g_first);
+        if (pr[q].g_rest) (void)hipGraphExecDestroy(pr[q].g_rest);
+      }
     }
-  } guard{&g_first, &g_rest};
-  while (done < nhist) {
-    uint64_t n = std::min<uint64_t>((uint64_t)chunk, nhist - done);
-    uint32_t bfirst = 0;
-    int nrows = 1;
-    if (nbatch > 0) {
-      // whole batches per chunk: at least one, as many as fit in `chunk` rows
-      const uint64_t b0 = next_batch;
-      uint64_t b1 = b0 + 1;
-      while (b1 < (uint64_t)nbatch && (int)(b1 - b0) < nrows_per_chunk &&
-             batch_start(b1 + 1, nhist, nbatch) - batch_start(b0, nhist, nbatch) <= (uint64_t)chunk)
-        ++b1;
-      n = batch_start(b1, nhist, nbatch) - batch_start(b0, nhist, nbatch);
-      bfirst = (uint32_t)b0;
-      nrows = (int)(b1 - b0);
-      next_batch = b1;
+  } guard{pr};
+
+  HIP_TRY(hipEventRecord(t->ev[0], s));
+  // the second pipe starts after what the caller enqueued on the run stream
+  HIP_TRY(hipEventRecord(t->ev[2], s));
+  HIP_TRY(hipStreamWaitEvent(t->pipe_stream, t->ev[2], 0));
+
+  // rps rounds of photons then electrons; the first generates the chunk's
+  // primaries. Round r reads queues `cur` and appends photons to `nxt`;
+  // electrons of both kernels go to the electron queue the electron kernel
+  // of the round (photons) or the next round (electrons) reads.
+  auto enqueue = [&](PipeRun& u, bool with_primaries, bool timed) -> hipError_t {
+    const Chunk& ch = chunks[u.chunk];
+    const PrimRange none{0, 0, ch.bfirst};
+    QueueView<R>* q = u.ws->q;
+    int c = 0;
+    for (int k = 0; k < rps; ++k) {
+      const int nx = c ^ 1;
+      hipLaunchKernelGGL(k_round_prep, dim3(1), dim3(64), 0, u.s, u.ctl, nx);
+      if (timed) (void)hipEventRecord(u.ev[4 * k], u.s);
+      u.P.kind = 0;
+      hipError_t e = Launch<R>::photons(u.P, dS, with_primaries && k == 0 ? ch.r_ph : none, p->geom, blocks, q[0 + c],
+                                        u.ctl + kCtlHeadG, q[2 + c], q[0 + nx], u.s);
+      if (e != hipSuccess) return e;
+      if (timed) { (void)hipEventRecord(u.ev[4 * k + 1], u.s); (void)hipEventRecord(u.ev[4 * k + 2], u.s); }
+      u.P.kind = 1;
+      e = Launch<R>::electrons(u.P, dS, with_primaries && k == 0 ? ch.r_el : none, p->geom, blocks, q[2 + c],
+                               u.ctl + kCtlHeadE, q[2 + nx], q[0 + nx], u.s);
+      if (e != hipSuccess) return e;
+      if (timed) (void)hipEventRecord(u.ev[4 * k + 3], u.s);
+      c = nx;
     }
-    // this chunk's primaries in each kind's index space
-    const uint64_t a = done, b = done + n;
-    const PrimRange r_ph{std::min(a, n_ph), std::min(b, n_ph), bfirst};
-    const PrimRange r_el{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst};
-    const PrimRange none{0, 0, bfirst};
-    HIP_TRY(hipMemsetAsync(t->ctl, 0, kCtlInts * sizeof(int), s));
-    // rps rounds of photons then electrons; the first generates
-    // this chunk's primaries. Round r reads queues `cur` and appends photons
-    // to `nxt`; electrons of both kernels go to the electron queue the
-    // electron kernel of the round (photons) or the next round (electrons) reads.
-    auto enqueue = [&](bool with_primaries, bool timed) -> hipError_t {
-      int c = 0;
+    return hipSuccess;
+  };
+  std::deque<int> inflight;   // pipes with a block of rounds enqueued, oldest first
+  // one block of rps rounds on pipe q, then the copy of its control words
+  auto launch = [&](int q) -> int {
+    PipeRun& u = pr[q];
+    if (use_graph) {
+      // the first block of a chunk generates its primaries (a graph per
+      // chunk); later blocks replay one graph per pipe
+      hipGraphExec_t& ge = u.first ? u.g_first : u.g_rest;
+      if (u.first && u.g_first) { (void)hipGraphExecDestroy(u.g_first); u.g_first = nullptr; }
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(u.s, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = enqueue(u, u.first, false);
+        const hipError_t e2 = hipStreamEndCapture(u.s, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return set_error(DPM_E_DEVICE, "graph capture of the transport rounds failed");
+        }
+        const hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) { ge = nullptr; return set_error(DPM_E_DEVICE, "graph instantiation failed"); }
+        ++n_graphs;
+      }
+      HIP_TRY(hipGraphLaunch(ge, u.s));
+    } else {
+      HIP_TRY(enqueue(u, u.first, true));
+    }
+    HIP_TRY(hipMemcpyAsync((void*)u.host, u.ctl, kCtlInts * sizeof(int), hipMemcpyDeviceToHost, u.s));
+    HIP_TRY(hipEventRecord(u.ev[kDone], u.s));
+    inflight.push_back(q);
+    return DPM_OK;
+  };
+  size_t next_chunk = 0;
+  auto start = [&](int q) -> int {
+    PipeRun& u = pr[q];
+    u.chunk = next_chunk++;
+    u.first = true;
+    HIP_TRY(hipMemsetAsync(u.ctl, 0, kCtlInts * sizeof(int), u.s));
+    return launch(q);
+  };
+  for (int q = 0; q < npipes && next_chunk < chunks.size(); ++q)
+    if ((rc = start(q)) != DPM_OK) return rc;
+  bool fold_enqueued = false;
+  while (!inflight.empty()) {
+    const int q = inflight.front();
+    inflight.pop_front();
+    PipeRun& u = pr[q];
+    HIP_TRY(hipEventSynchronize(u.ev[kDone]));
+    u.first = false;
+    if (!use_graph)
       for (int k = 0; k < rps; ++k) {
-        const int nx = c ^ 1;
-        hipLaunchKernelGGL(k_round_prep, dim3(1), dim3(64), 0, s, t->ctl, nx);
-        if (timed) (void)hipEventRecord(t->ev[10 + 4 * k], s);
-        P.kind = 0;
-        hipError_t e = Launch<R>::photons(P, dS, with_primaries && k == 0 ? r_ph : none, p->geom, blocks,
-                                          ws.q[0 + c], t->ctl + kCtlHeadG, ws.q[2 + c], ws.q[0 + nx], s);
-        if (e != hipSuccess) return e;
-        if (timed) { (void)hipEventRecord(t->ev[11 + 4 * k], s); (void)hipEventRecord(t->ev[12 + 4 * k], s); }
-        P.kind = 1;
-        e = Launch<R>::electrons(P, dS, with_primaries && k == 0 ? r_el : none, p->geom, blocks, ws.q[2 + c],
-                                 t->ctl + kCtlHeadE, ws.q[2 + nx], ws.q[0 + nx], s);
-        if (e != hipSuccess) return e;
-        if (timed) (void)hipEventRecord(t->ev[13 + 4 * k], s);
-        c = nx;
+        ms_ph += elapsed(u.ev[4 * k], u.ev[4 * k + 1]);
+        ms_el += elapsed(u.ev[4 * k + 2], u.ev[4 * k + 3]);
+        ++c_ph; ++c_el;
       }
-      return hipSuccess;
-    };
-    bool first = true;
-    for (;;) {
-      if (use_graph) {
-        hipGraphExec_t& ge = first ? g_first : g_rest;
-        if (first && g_first) { (void)hipGraphExecDestroy(g_first); g_first = nullptr; }
-        if (!ge) {
-          hipGraph_t g = nullptr;
-          HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-          const hipError_t e = enqueue(first, false);
-          const hipError_t e2 = hipStreamEndCapture(s, &g);
-          if (e != hipSuccess || e2 != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return set_error(DPM_E_DEVICE, "graph capture of the transport rounds failed");
-          }
-          const hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-          (void)hipGraphDestroy(g);
-          if (e3 != hipSuccess) { ge = nullptr; return set_error(DPM_E_DEVICE, "graph instantiation failed"); }
-          ++n_graphs;
-        }
-        HIP_TRY(hipGraphLaunch(ge, s));
-      } else {
-        HIP_TRY(enqueue(first, true));
-      }
-      int counts[kCtlInts];
-      HIP_TRY(hipMemcpyAsync(counts, t->ctl, kCtlInts * sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      first = false;
-      if (!use_graph)
-        for (int k = 0; k < rps; ++k) {
-          ms_ph += elapsed(t->ev[10 + 4 * k], t->ev[11 + 4 * k]);
-          ms_el += elapsed(t->ev[12 + 4 * k], t->ev[13 + 4 * k]);
-          ++c_ph; ++c_el;
-        }
-      rounds += rps;
-      if (counts[kCtlOverflow] != 0)
-        return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
-                                             "); use a smaller chunk");
-      // an even number of rounds leaves the next inputs in queues 0 and 2
-      // (packed counters: ints 0-1 and 4-5)
-      if ((counts[0] | counts[1] | counts[4] | counts[5]) == 0) break;
-      if (rounds > 100000) return set_error(DPM_E_DEVICE, "transport did not drain");
+    if (u.fold_timed) {   // the pipe's previous fold preceded this block on its stream
+      ms_fold += elapsed(u.ev[kFold0], u.ev[kFold1]);
+      u.fold_timed = false;
+    }
+    rounds += rps;
+    if (u.host[kCtlOverflow] != 0)
+      return set_error(DPM_E_OVERFLOW, "particle queue overflow (capacity " + std::to_string(cap) +
+                                           "); use a smaller chunk");
+    if (rounds > 100000) return set_error(DPM_E_DEVICE, "transport did not drain");
+    // an even number of rounds leaves the next inputs in queues 0 and 2
+    // (packed counters: ints 0-1 and 4-5)
+    if ((u.host[0] | u.host[1] | u.host[4] | u.host[5]) != 0) {
+      if ((rc = launch(q)) != DPM_OK) return rc;
+      continue;
     }
     if (nbatch > 0) {
-      HIP_TRY(hipEventRecord(t->ev[7], s));
-      hipLaunchKernelGGL(k_fold, dim3(std::max(1, t->num_cu * 8)), dim3(256), 0, s, d_rows, nrows, p->nvox, d_sum,
-                         d_sum2);
+      // folds add into the shared sum and sum of squares: one at a time
+      if (fold_enqueued) HIP_TRY(hipStreamWaitEvent(u.s, t->ev[3], 0));
+      HIP_TRY(hipEventRecord(u.ev[kFold0], u.s));
+      hipLaunchKernelGGL(k_fold, dim3(std::max(1, t->num_cu * 8)), dim3(256), 0, u.s, u.P.tally,
+                         chunks[u.chunk].nrows, p->nvox, d_sum, d_sum2);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventRecord(t->ev[8], s));
-      HIP_TRY(hipEventSynchronize(t->ev[8]));
-      ms_fold += elapsed(t->ev[7], t->ev[8]);
+      HIP_TRY(hipEventRecord(u.ev[kFold1], u.s));
+      HIP_TRY(hipEventRecord(t->ev[3], u.s));
+      fold_enqueued = true;
+      u.fold_timed = true;
       ++c_fold;
     }
-    done += n;
+    if (next_chunk < chunks.size() && (rc = start(q)) != DPM_OK) return rc;
   }
-  HIP_TRY(hipEventRecord(t->ev[9], s));
-  HIP_TRY(hipEventSynchronize(t->ev[9]));
+  // the run stream waits for the second pipe
+  HIP_TRY(hipEventRecord(t->ev[2], t->pipe_stream));
+  HIP_TRY(hipStreamWaitEvent(s, t->ev[2], 0));
+  HIP_TRY(hipEventRecord(t->ev[1], s));
+  HIP_TRY(hipEventSynchronize(t->ev[1]));
+  for (int q = 0; q < kPipes; ++q)
+    if (pr[q].fold_timed) ms_fold += elapsed(pr[q].ev[kFold0], pr[q].ev[kFold1]);
   if (st) {
-    st->seconds = elapsed(t->ev[0], t->ev[9]) * 1e-3;
+    st->seconds = elapsed(t->ev[0], t->ev[1]) * 1e-3;
     st->rounds = rounds;
     // slot 0 (source) stays 0: primaries are generated inside the transport kernels
     st->kernel_ms[0] = 0; st->kernel_ms[1] = ms_ph; st->kernel_ms[2] = ms_el; st->kernel_ms[3] = ms_fold;
@@ -1059,13 +1201,20 @@ int dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out) 
     dpm_tables_destroy(t);
     return rc;
   }
+  // both streams at the default priority
   if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&t->ctl, kCtlInts * sizeof(int)) != hipSuccess || hipMalloc(&t->stats, sizeof(StatsDev)) != hipSuccess) {
+      hipStreamCreateWithFlags(&t->pipe_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&t->ctl, kPipes * kCtlInts * sizeof(int)) != hipSuccess ||
+      hipHostMalloc(&t->ctl_host, kPipes * kCtlInts * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&t->stats, sizeof(StatsDev)) != hipSuccess) {
     dpm_tables_destroy(t);
-    return set_error(DPM_E_DEVICE, "cannot create the run stream or control block");
+    return set_error(DPM_E_DEVICE, "cannot create the run streams or control block");
   }
   for (auto& e : t->ev)
     if (hipEventCreate(&e) != hipSuccess) { dpm_tables_destroy(t); return set_error(DPM_E_DEVICE, "event creation failed"); }
+  for (auto& pe : t->pev)
+    for (auto& e : pe)
+      if (hipEventCreate(&e) != hipSuccess) { dpm_tables_destroy(t); return set_error(DPM_E_DEVICE, "event creation failed"); }
   *out = t;
   return DPM_OK;
 }
@@ -1090,18 +1239,26 @@ void dpm_tables_destroy(dpm_tables* t) {
   if (!t) return;
   (void)hipSetDevice(t->device);
   if (t->stream) (void)hipStreamSynchronize(t->stream);
-  t->wsf.release();
-  t->wsd.release();
+  if (t->pipe_stream) (void)hipStreamSynchronize(t->pipe_stream);
+  for (int q = 0; q < kPipes; ++q) {
+    t->wsf[q].release();
+    t->wsd[q].release();
+  }
   for (DevBuf* b : {&t->buf_rows, &t->buf_steps, &t->buf_sum, &t->buf_sum2, &t->buf_he, &t->buf_src, &t->buf_blaux}) b->release();
   if (t->csr_col) (void)hipFree(t->csr_col);
   if (t->csr_val) (void)hipFree(t->csr_val);
   if (t->dev_f) (void)hipFree(t->dev_f);
   if (t->dev_d) (void)hipFree(t->dev_d);
   if (t->ctl) (void)hipFree(t->ctl);
+  if (t->ctl_host) (void)hipHostFree(t->ctl_host);
   if (t->stats) (void)hipFree(t->stats);
   for (auto& e : t->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& pe : t->pev)
+    for (auto& e : pe)
+      if (e) (void)hipEventDestroy(e);
   if (t->stream) (void)hipStreamDestroy(t->stream);
+  if (t->pipe_stream) (void)hipStreamDestroy(t->pipe_stream);
   delete t;
 }
 
@@ -1178,6 +1335,7 @@ int dpm_simulate(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uin
   if (opt_in) opt = *opt_in;
   if (opt.hist_begin + nhist > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
   if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (opt.pipes < 0 || opt.pipes > kPipes) return set_error(DPM_E_INVALID, "pipes must be 0, 1 or 2");
   if (opt.nbatch > 1 && (uint64_t)opt.nbatch > nhist) return set_error(DPM_E_INVALID, "more batches than histories");
   const int rc = validate_source(t, p, src);
   if (rc != DPM_OK) return rc;
@@ -1193,6 +1351,7 @@ int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* s
   std::memset(&opt, 0, sizeof(opt));
   if (opt_in) opt = *opt_in;
   if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (opt.pipes < 0 || opt.pipes > kPipes) return set_error(DPM_E_INVALID, "pipes must be 0, 1 or 2");
   int rc = validate_source(t, p, src);
   if (rc != DPM_OK) return rc;
   HostSource hs = source_from(src, 1, 0);
@@ -1220,13 +1379,9 @@ static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource
   const bool slab = p->geom == GEOM_SLAB;
   const int nbatch = (!slab && opt.nbatch > 1) ? opt.nbatch : 0;
   const int64_t nb = slab ? p->nbins : p->nvox;
-  // tally rows: one, or the batches of one chunk
-  int64_t chunk = opt.chunk > 0 ? opt.chunk : kDefaultChunk;
-  int rows = 1;
-  if (nbatch > 0) {
-    const uint64_t per = nhist / nbatch + 1;
-    rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)chunk / per));
-  }
+  // tally rows: one, or the batches of one chunk for each pipe
+  const ChunkPlan plan = plan_chunks(nhist, nbatch, opt.chunk, pipe_count(&opt));
+  const int rows = nbatch > 0 ? plan.rows * pipe_count(&opt) : 1;
   if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (slab && !t->buf_steps.ensure(sizeof(double) * nb)) ||
       (nbatch > 0 && (!t->buf_sum.ensure(sizeof(double) * nb) || !t->buf_sum2.ensure(sizeof(double) * nb))) ||
       (opt.hist_edep && !t->buf_he.ensure(sizeof(double) * nhist)))
@@ -1250,8 +1405,8 @@ static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource
   dpm_stats local;
   std::memset(&local, 0, sizeof(local));
   rc = opt.precision == DPM_FP64
-           ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local)
-           : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, d_rows, rows, d_steps, d_sum, d_sum2, d_he, &local);
+           ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, plan, d_rows, d_steps, d_sum, d_sum2, d_he, &local)
+           : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, plan, d_rows, d_steps, d_sum, d_sum2, d_he, &local);
   if (rc == DPM_OK) {
     // scored energy: reduced on the device from the tally that is returned
     hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, nbatch > 0 ? d_sum : d_rows, nb,
@@ -1299,6 +1454,7 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   std::memset(&opt, 0, sizeof(opt));
   if (opt_in) opt = *opt_in;
   if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (opt.pipes < 0 || opt.pipes > kPipes) return set_error(DPM_E_INVALID, "pipes must be 0, 1 or 2");
   // one source entry per beamlet; its histories keep the ids they have when
   // the beamlets run one after the other from hist_begin
   HostSource hs;
@@ -1337,9 +1493,10 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   dpm_stats local;
   std::memset(&local, 0, sizeof(local));
   opt.nbatch = 0;
+  const ChunkPlan plan = plan_chunks(total, 0, opt.chunk, pipe_count(&opt));
   int rc = opt.precision == DPM_FP64
-               ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local)
-               : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, d_rows, 1, nullptr, nullptr, nullptr, nullptr, &local);
+               ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, plan, d_rows, nullptr, nullptr, nullptr, nullptr, &local)
+               : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, plan, d_rows, nullptr, nullptr, nullptr, nullptr, &local);
   if (rc == DPM_OK) {
     hipLaunchKernelGGL(k_bl_stats, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax, &t->stats->scored);
     if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "beamlet row statistics launch failed");
@@ -1394,19 +1551,25 @@ namespace {
 // Times every candidate schedule on the sample source `hs` (device time of
 // run_transport, best of two runs) and writes the fastest to *out.
 template <typename R>
-int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int rows, int nbatch, Workspace<R>& ws,
+int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int rows, int nbatch, Workspace<R>* ws,
                   Tables<R>& T, dpm_schedule* out) {
   const uint64_t n = hs.count(0) + hs.count(1);
   const int64_t nb = p->geom == GEOM_SLAB ? p->nbins : p->nvox;
   // batch statistics as the real run keeps them: the batch rows of a chunk
   // (their footprint sets the cost of the tally atomics) and the fold
   if (p->geom == GEOM_SLAB || nbatch < 2 || (uint64_t)nbatch > n) nbatch = 0;
-  if (nbatch > 0) {
-    const uint64_t per = n / nbatch + 1;
-    rows = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nbatch, (uint64_t)kDefaultChunk / per));
+  // the chunks of a real run of the sample (the automatic plan of each pipe count)
+  ChunkPlan plan[kPipes + 1];
+  int rows_of[kPipes + 1];
+  int rows_max = rows;
+  for (int np = 1; np <= kPipes; ++np) {
+    plan[np] = plan_chunks(n, nbatch, 0, np);
+    rows_of[np] = nbatch > 0 ? plan[np].rows * np : rows;
+    rows_max = std::max(rows_max, rows_of[np]);
   }
   t->rows_zero = 0;   // candidates leave their tallies in the rows
-  if (!t->buf_rows.ensure(sizeof(double) * nb * rows) || (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)) ||
+  if (!t->buf_rows.ensure(sizeof(double) * nb * rows_max) ||
+      (p->geom == GEOM_SLAB && !t->buf_steps.ensure(sizeof(double) * nb)) ||
       (nbatch > 0 && (!t->buf_sum.ensure(sizeof(double) * nb) || !t->buf_sum2.ensure(sizeof(double) * nb))))
     return set_error(DPM_E_NOMEM, "cannot allocate the tally");
   static const int kWaves[] = {8, 12, 16};
@@ -1416,43 +1579,46 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
   std::memset(&best, 0, sizeof(best));
   double best_s = 1e300, default_s = 0.0;
   int cand = 0;
-  for (int w : kWaves)
-    for (int r : kRefill)
-      for (int tm : kTally) {
-        if (tm != 0 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
-        dpm_run_options opt;
-        std::memset(&opt, 0, sizeof(opt));
-        opt.seed = 0x5eed;
-        opt.hist_begin = 0;
-        opt.chunk = (int64_t)n;
-        opt.nbatch = nbatch;
-        opt.waves_per_cu = w;
-        opt.refill_min = r;
-        opt.tally_mode = tm;
-        double sec = 1e300;
-        for (int rep = 0; rep < 2; ++rep) {
-          (void)hipMemsetAsync(t->buf_rows.p, 0, sizeof(double) * nb * rows, t->stream);
-          if (p->geom == GEOM_SLAB) (void)hipMemsetAsync(t->buf_steps.p, 0, sizeof(double) * nb, t->stream);
-          (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream);
-          dpm_stats st;
-          std::memset(&st, 0, sizeof(st));
-          const int rc = run_transport<R>(t, p, hs, &opt, ws, T, t->buf_rows.d(), rows,
-                                          p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr,
-                                          nbatch > 0 ? t->buf_sum.d() : nullptr, nbatch > 0 ? t->buf_sum2.d() : nullptr,
-                                          nullptr, &st);
-          if (rc != DPM_OK) return rc;
-          sec = std::min(sec, st.seconds);
+  for (int np = 1; np <= kPipes; ++np)
+    for (int w : kWaves)
+      for (int r : kRefill)
+        for (int tm : kTally) {
+          if (tm != 0 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
+          dpm_run_options opt;
+          std::memset(&opt, 0, sizeof(opt));
+          opt.seed = 0x5eed;
+          opt.hist_begin = 0;
+          opt.chunk = 0;
+          opt.nbatch = nbatch;
+          opt.waves_per_cu = w;
+          opt.refill_min = r;
+          opt.tally_mode = tm;
+          opt.pipes = np;
+          double sec = 1e300;
+          for (int rep = 0; rep < 2; ++rep) {
+            (void)hipMemsetAsync(t->buf_rows.p, 0, sizeof(double) * nb * rows_of[np], t->stream);
+            if (p->geom == GEOM_SLAB) (void)hipMemsetAsync(t->buf_steps.p, 0, sizeof(double) * nb, t->stream);
+            (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream);
+            dpm_stats st;
+            std::memset(&st, 0, sizeof(st));
+            const int rc = run_transport<R>(t, p, hs, &opt, ws, T, plan[np], t->buf_rows.d(),
+                                            p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr,
+                                            nbatch > 0 ? t->buf_sum.d() : nullptr,
+                                            nbatch > 0 ? t->buf_sum2.d() : nullptr, nullptr, &st);
+            if (rc != DPM_OK) return rc;
+            sec = std::min(sec, st.seconds);
+          }
+          ++cand;
+          if (np == 1 && w == 16 && r == 1 && tm == 0) default_s = sec;
+          if (sec < best_s) {
+            best_s = sec;
+            best.waves_per_cu = w;
+            best.refill_min = r;
+            best.tally_mode = tm;
+            best.pipes = np;
+          }
         }
-        ++cand;
-        if (w == 16 && r == 1 && tm == 0) default_s = sec;
-        if (sec < best_s) {
-          best_s = sec;
-          best.waves_per_cu = w;
-          best.refill_min = r;
-          best.tally_mode = tm;
-        }
-      }
-  best.chunk = kDefaultChunk;
+  best.chunk = 0;   // the automatic chunk plan of the chosen pipe count, as timed
   best.candidates = cand;
   best.histories_per_s = (double)n / best_s;
   best.default_histories_per_s = default_s > 0 ? (double)n / default_s : 0.0;

@@ -49,7 +49,8 @@ class RunOptions(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("hist_begin", ctypes.c_uint64), ("precision", ctypes.c_int),
                 ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
                 ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int),
-                ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int), ("table_memory", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int), ("table_memory", ctypes.c_int),
+                ("pipes", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -74,7 +75,7 @@ class Stats(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int), ("refill_min", ctypes.c_int),
                 ("tally_mode", ctypes.c_int), ("candidates", ctypes.c_int), ("histories_per_s", ctypes.c_double),
-                ("default_histories_per_s", ctypes.c_double)]
+                ("default_histories_per_s", ctypes.c_double), ("pipes", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -258,7 +259,7 @@ def pinned_empty(n, dtype=np.float64):
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
              out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
-             angle=None, emode=None, table_memory=0):
+             angle=None, emode=None, table_memory=0, pipes=0):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -272,9 +273,11 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
                                                       schedule["refill_min"], schedule["tally_mode"])
+        pipes = schedule.get("pipes", 0)
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, chunk=chunk,
                      waves_per_cu=waves_per_cu, hist_edep=_ptr(he), refill_min=refill_min, tally_mode=tally_mode,
-                     graph=int(bool(graph)), rounds_per_sync=rounds_per_sync, table_memory=table_memory)
+                     graph=int(bool(graph)), rounds_per_sync=rounds_per_sync, table_memory=table_memory,
+                     pipes=pipes)
     if out is not None:
         tally, steps = out
         for a in (tally, steps):
@@ -305,15 +308,17 @@ def _source(particle, energy, emin, emax, field, x0, y0, angle, emode):
 
 def simulate_fields(tables, phantom, fields, particle=PHOTON, energy=10.0, emin=0.0, emax=0.0, emode=None,
                     field=100.0, gantry=True, seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, schedule=None,
-                    out=None, fetch=True, table_memory=0):
+                    out=None, fetch=True, table_memory=0, pipes=0):
     """fields: list of (x0, y0, angle_deg, nhist); one run into one tally
     (dpm_simulate_fields). Returns (tally, sum2_or_steps, stats dict)."""
     src = _source(particle, energy, emin, emax, field, 0.0, 0.0, 0.0 if gantry else None, emode)
     arr = (Field * len(fields))(*[Field(x0=f[0], y0=f[1], angle_deg=f[2], nhist=int(f[3])) for f in fields])
-    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, table_memory=table_memory)
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, table_memory=table_memory,
+                     pipes=pipes)
     if schedule is not None:
         opt.chunk, opt.waves_per_cu = schedule["chunk"], schedule["waves_per_cu"]
         opt.refill_min, opt.tally_mode = schedule["refill_min"], schedule["tally_mode"]
+        opt.pipes = schedule.get("pipes", 0)
     st = Stats()
     if not fetch:
         _check(lib().dpm_simulate_fields(tables.h, phantom.h, ctypes.byref(src), arr, len(fields), ctypes.byref(opt),
@@ -359,16 +364,17 @@ def autotune_beamlets(tables, phantom, beamlets, nsample=4000000, precision=DPM_
 
 def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
                       waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0, graph=True,
-                      table_memory=0):
+                      table_memory=0, pipes=0):
     """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
     arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
                                        for b in beamlets])
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
                                                       schedule["refill_min"], schedule["tally_mode"])
+        pipes = schedule.get("pipes", 0)
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
                      refill_min=refill_min, tally_mode=tally_mode, graph=int(bool(graph)),
-                     table_memory=table_memory)
+                     table_memory=table_memory, pipes=pipes)
     row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
     st = Stats()
     _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,

@@ -156,7 +156,8 @@ typedef struct dpm_run_options {
   uint64_t hist_begin;             /* id of the first history (stream selection) */
   int precision;                   /* DPM_FP32 (default) or DPM_FP64 */
   int nbatch;                      /* >1: per-batch sum and sum of squares of the voxel tally */
-  int64_t chunk;                   /* histories per queue fill; 0 = automatic */
+  int64_t chunk;                   /* histories per queue fill; 0 = automatic (one pipe: up to 16M
+                                      histories; two pipes: an even number of chunks of up to 8M) */
   int waves_per_cu;                /* persistent waves per CU; 0 = automatic */
   double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
   int refill_min;                  /* idle lanes of a wave that trigger its refill (1..64); 0 = 1 */
@@ -172,6 +173,10 @@ typedef struct dpm_run_options {
                                       IMFP tables (2+3*nmat arrays of 1024 values: up to 3 materials);
                                       DPM_TABLES_GLOBAL (1): read them from device memory. Both give
                                       identical results. */
+  int pipes;                       /* 1 (0 = 1) or 2: chunks run on that many pipes -- streams with
+                                      their own particle queues and batch rows -- side by side, so
+                                      that one pipe's kernel tails, drain rounds and folds overlap the
+                                      other's transport; the same results either way */
 } dpm_run_options;
 
 enum { DPM_TABLES_AUTO = 0, DPM_TABLES_GLOBAL = 1 };
@@ -261,9 +266,9 @@ int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val);
  * CU, refill threshold (compaction interval) and voxel-tally mode -- on a
  * sample of `nsample` histories of the given source (or beamlets, scaled to
  * about nsample histories in total) and returns the fastest. The chunk is
- * not sampled (larger chunks always shorten the drain tails): `chunk` is set
- * to the largest the engine allows; callers clamp it to their run size.
- * Copy the fields into dpm_run_options for the real run.
+ * not sampled: candidates run with the automatic chunk plan of their pipe
+ * count, and `chunk` is returned as 0 (automatic). Copy the fields into
+ * dpm_run_options for the real run.
  * ------------------------------------------------------------------------- */
 typedef struct dpm_schedule {
   int64_t chunk;
@@ -273,6 +278,7 @@ typedef struct dpm_schedule {
   int candidates;                  /* schedules timed */
   double histories_per_s;          /* device rate of the best on the sample */
   double default_histories_per_s;  /* device rate of the default schedule on the sample */
+  int pipes;                       /* 1 or 2 (dpm_run_options.pipes) */
 } dpm_schedule;
 
 /* nbatch (> 1, voxel phantoms): time the candidates with the batch

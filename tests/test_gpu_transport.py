@@ -175,7 +175,9 @@ def test_schedules_do_not_change_results(gpu_tables0):
     for sched in (dict(chunk=1000, waves_per_cu=8, refill_min=16, tally_mode=1),
                   dict(chunk=0, waves_per_cu=12, refill_min=4, tally_mode=0),
                   dict(chunk=0, waves_per_cu=12, refill_min=4, tally_mode=2),
-                  dict(chunk=0, waves_per_cu=16, refill_min=64, tally_mode=1)):
+                  dict(chunk=0, waves_per_cu=16, refill_min=64, tally_mode=1),
+                  dict(chunk=0, waves_per_cu=8, refill_min=4, tally_mode=0, pipes=2),
+                  dict(chunk=700, waves_per_cu=12, refill_min=1, tally_mode=2, pipes=2)):
         d, _, _, h = dpm_amd.simulate(gpu_tables0, ph, 4000, schedule=sched, **kw)
         np.testing.assert_allclose(h, h0, rtol=1e-9, atol=1e-9)
         np.testing.assert_allclose(d, d0, rtol=1e-9, atol=1e-12)
@@ -187,16 +189,49 @@ def test_autotune_returns_a_timed_candidate(gpu_tables0):
     ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
     s = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0)
     assert s["waves_per_cu"] in (8, 12, 16) and s["refill_min"] in (1, 4, 16) and s["tally_mode"] in (0, 1, 2)
-    assert s["candidates"] == 27 and s["chunk"] > 0
+    # the candidates ran with the automatic chunk plan (chunk 0) on one and
+    # on two pipes
+    assert s["candidates"] == 54 and s["chunk"] == 0 and s["pipes"] in (1, 2)
     assert s["histories_per_s"] >= s["default_histories_per_s"] > 0
     bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000)]
     sb = dpm_amd.autotune_beamlets(gpu_tables0, ph, bl, nsample=5000)
-    assert sb["candidates"] == 27 and sb["histories_per_s"] > 0
+    assert sb["candidates"] == 54 and sb["histories_per_s"] > 0
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.autotune(gpu_tables0, ph, nsample=10, particle=0, emin=0.5, emax=20.0)
     # with the batch rows and folds of a 10-batch run
     s10 = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0, nbatch=10)
-    assert s10["candidates"] == 27 and s10["histories_per_s"] >= s10["default_histories_per_s"] > 0
+    assert s10["candidates"] == 54 and s10["histories_per_s"] >= s10["default_histories_per_s"] > 0
+
+
+@pytest.mark.gpu
+def test_two_pipes_equal_one_pipe(gpu_tables0):
+    # two pipes: each keeps its own batch rows and folds them one at a time
+    # into the shared sum and sum of squares; chunks alternate between them
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    kw = dict(particle=0, emin=0.5, emax=20.0, field=40.0, seed=41, per_history=True, nbatch=10)
+    d1, s1, st1, h1 = dpm_amd.simulate(gpu_tables0, ph, 20000, **kw)
+    for extra in (dict(pipes=2), dict(pipes=2, chunk=3000), dict(pipes=2, chunk=3000, graph=True, rounds_per_sync=2)):
+        d2, s2, st2, h2 = dpm_amd.simulate(gpu_tables0, ph, 20000, **kw, **extra)
+        np.testing.assert_allclose(h2, h1, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(d2, d1, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(s2, s1, rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(st2["e_primary"], st1["e_primary"], rtol=1e-12)
+        assert st2["counters"]["e_steps"] == st1["counters"]["e_steps"]
+    # FP64 transport, and a plain (unbatched) tally both pipes add into
+    kw64 = dict(particle=-1, energy=12.0, field=40.0, seed=5, per_history=True, precision=dpm_amd.DPM_FP64)
+    a1, _, _, g1 = dpm_amd.simulate(gpu_tables0, ph, 4000, **kw64)
+    a2, _, _, g2 = dpm_amd.simulate(gpu_tables0, ph, 4000, pipes=2, chunk=900, **kw64)
+    np.testing.assert_allclose(g2, g1, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(a2, a1, rtol=1e-12, atol=1e-12)
+    # beamlets: rows chosen per history, shared by both pipes
+    bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000), (0.0, 0.0, 1, 15.0, 2000)]
+    ra, ca, va, _ = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, seed=8)
+    rb, cb, vb, _ = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, seed=8, pipes=2)
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    np.testing.assert_allclose(va, vb, rtol=1e-9)
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.simulate(gpu_tables0, ph, 100, particle=0, energy=5.0, pipes=3)
 
 
 @pytest.mark.gpu
