@@ -195,6 +195,95 @@ def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=40
                       % (cores, n_each, max(secs), wall, n_each / (sum(secs) / cores))}
 
 
+def _read_hist_sim(path):
+    """Mean deposit and the (depth, dose, steps) rows of a hist.sim file (Geom.cc:167-192 layout)."""
+    import numpy as np
+    mean, rows = None, []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                if "Mean energy deposit" in line:
+                    mean = float(line.split(":")[1].split()[0])
+                continue
+            if line.strip():
+                rows.append([float(v) for v in line.split()[1:4]])
+    return mean, np.array(rows)
+
+
+def config1_leg(dpm_amd, gamma, tables3, cpu=True, n=100_000):
+    """Configuration 1, the reference's own case: slab of configuration index 3
+    (Water/Ti/Bone/Water) at 1 mm, 15 MeV e- pencil beam, 1e5 histories. The
+    GPU runs the plain drop-in path (FP32 transport, FP64 depth tally); the CPU
+    baseline is the reference itself, compiled from its sources into
+    oracle/_ref/ (single-threaded, mt19937 seed 123 fixed in its code), run on
+    the same table files. Parity: 1 %/1 mm gamma of the two hist.sim depth-dose
+    curves (independent random streams, equal history count)."""
+    import tempfile
+    import numpy as np
+    t = dpm_amd.Tables(tables3)
+    ph = dpm_amd.Phantom.slab(t, 1.0, 3)
+    kw = dict(particle=dpm_amd.ELECTRON, energy=15.0, seed=1)
+    dpm_amd.simulate(t, ph, 2000, **kw)   # warm-up
+    walls = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        dpm_amd.simulate(t, ph, n, fetch=False, **kw)
+        walls.append(time.perf_counter() - t0)
+    ph.close()
+    t.close()
+    wall = float(np.median(walls))
+    r = {"workload": "config1: slab of configuration index 3 (Water/Ti/Bone/Water), 1 mm, 15 MeV e- pencil beam, "
+                     "%d histories, FP32 transport, FP64 depth tally" % n,
+         "histories_per_s": n / wall, "seconds": wall}
+    d = tempfile.mkdtemp(prefix="dpm_c1_")
+    gpu_out, ref_out = os.path.join(d, "gpu.sim"), os.path.join(d, "ref.sim")
+    args = ["-p", "e-", "-e", "15", "-n", str(n), "-d", tables3, "-b", "1.0", "-c", "3"]
+    try:
+        # the drop-in command line (same flags as the reference's), end to end
+        t0 = time.perf_counter()
+        subprocess.check_call([os.path.join(ROOT, "dpm-g4cpp_amd", "bin", "dpm_Simulate")] + args +
+                              ["-o", gpu_out, "-s", "1"], stdout=subprocess.DEVNULL)
+        r["cli_seconds_end_to_end"] = time.perf_counter() - t0
+        ref_bin = os.path.join(ROOT, "oracle", "_ref", "dpm_Simulate_ref")
+        if not cpu:
+            return r
+        if not os.path.exists(ref_bin):
+            r["cpu_baseline"] = {"error": "oracle/_ref/dpm_Simulate_ref not built (needs the reference sources)"}
+            return r
+        t0 = time.perf_counter()
+        subprocess.check_call([ref_bin] + args + ["-o", ref_out], stdout=subprocess.DEVNULL)
+        single = time.perf_counter() - t0
+        # throughput of the host: one copy per core, side by side (the
+        # reference has no seed option, so every copy runs the same histories)
+        cores = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([ref_bin] + args + ["-o", os.path.join(d, "ref%d.sim" % i)],
+                                  stdout=subprocess.DEVNULL) for i in range(cores)]
+        rcs = [p.wait() for p in procs]
+        multi = time.perf_counter() - t0
+        if any(rcs):
+            raise RuntimeError("reference copies failed: %s" % rcs)
+        mg, hg = _read_hist_sim(gpu_out)
+        mr, hr = _read_hist_sim(ref_out)
+        m = min(len(hg), len(hr))
+        pr = gamma.gamma_pass_rate_1d(hr[:m, 1], hg[:m, 1], spacing_mm=1.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+        r["cpu_baseline"] = {"value": cores * n / multi, "unit": "histories/s", "cores": cores, "kind": "reference",
+                             "sample": "%d concurrent copies of the reference dpm_Simulate (compiled from its "
+                                       "sources, -O3), each the full %d histories, wall %.1f s incl. table loading; "
+                                       "single copy %.2f s = %.0f hist/s" % (cores, n, multi, single, n / single)}
+        r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
+        r["equal_history_depth_dose"] = {
+            "gamma_pass_rate_gpu_fp32_vs_reference": pr,
+            "criterion": "1%/1mm global, depth bins > 10% of the reference maximum, independent streams",
+            "mean_edep_MeV_gpu": mg, "mean_edep_MeV_reference": mr,
+            "max_abs_diff_rel_to_max": float(np.max(np.abs(hg[:m, 1] - hr[:m, 1])) / hr[:m, 1].max())}
+    finally:
+        for f in os.listdir(d):
+            os.remove(os.path.join(d, f))
+        os.rmdir(d)
+    return r
+
+
 def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     """Configuration 3: 256x256x128 heterogeneous phantom at 2 mm, 12 MeV e-
     10x10 cm2 field, 1e8 histories, FP32 transport, FP64 tally, 100 batches."""
@@ -210,7 +299,6 @@ def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
         # sample: 1e7 histories in batches of the real run's size (1e6)
         sched = dpm_amd.autotune(t, ph, nsample=10_000_000, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0,
                                  nbatch=10)
-        sched["chunk"] = 0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     dpm_amd.simulate(t, ph, 1_000_000, schedule=sched, out=out, fetch=False, **kw)   # warm-up
     t0 = time.perf_counter()
@@ -255,7 +343,6 @@ def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     else:
         sched = dpm_amd.autotune(t, ph, nsample=16_000_000, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
                                  field=100.0, angle=45.0, nbatch=2)
-        sched["chunk"] = 0
     dpm_amd.simulate_fields(t, ph, phantoms.config5_fields(1e7), fetch=False, schedule=sched, **kw)   # warm-up
     t0 = time.perf_counter()
     _, _, st = dpm_amd.simulate_fields(t, ph, plan, nbatch=100, fetch=False, schedule=sched, **kw)
@@ -283,6 +370,26 @@ def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     return r
 
 
+def voxel_error_report(d32, sum2, d64, nbatch, cutoff=0.1):
+    """Per-voxel error of the FP32-transport dose against the FP64-everywhere
+    run of the same histories, next to the dose's own statistical uncertainty
+    (standard error of the total from the batch sum and sum of squares), over
+    voxels above `cutoff` of the FP64 maximum."""
+    import numpy as np
+    m = d64 >= cutoff * d64.max()
+    diff = np.abs(d32[m] - d64[m])
+    rel = diff / d64[m]
+    var = np.maximum(sum2[m] - d32[m] * d32[m] / nbatch, 0.0) * nbatch / (nbatch - 1)
+    sig = np.sqrt(var)
+    z = diff / np.maximum(sig, 1e-300)
+    return {"voxels": int(m.sum()), "rel_err_median": float(np.median(rel)), "rel_err_p99": float(np.quantile(rel, 0.99)),
+            "rel_err_max": float(rel.max()), "stat_rel_sigma_median": float(np.median(sig / d32[m])),
+            "err_over_sigma_median": float(np.median(z)), "err_over_sigma_p99": float(np.quantile(z, 0.99)),
+            "frac_err_below_sigma": float(np.mean(z < 1.0)),
+            "what": "|FP32 - FP64| per voxel (same histories and random streams) vs the FP32 dose's batch "
+                    "standard error (%d batches)" % nbatch}
+
+
 def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, sched=None, plain=True):
     """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
     per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
@@ -294,7 +401,6 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
     total = sum(b[4] for b in bl)
     if sched is None:
         sched = dpm_amd.autotune_beamlets(t, ph, bl, nsample=4000000)
-        sched["chunk"] = 0
         sched["source"] = "dpm_autotune_beamlets on ~4e6 histories of this beamlet set"
     # warm-up: same beamlets with few histories (allocates the per-beamlet rows)
     dpm_amd.simulate_beamlets(t, ph, [b[:4] + (100,) for b in bl], rel_threshold=1e-3, seed=seed)
@@ -407,7 +513,6 @@ def main():
         sched = dpm_amd.autotune(t, ph, nsample=ns, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field,
                                  nbatch=args.nbatch)
         sched["source"] = "dpm_autotune on %d histories of this workload (%d batches)" % (ns, args.nbatch)
-        sched["chunk"] = 0
 
     def step(precision=dpm_amd.DPM_FP32, seed=1, fetch=True):
         return dpm_amd.simulate(t, ph, nhist, particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=seed,
@@ -477,7 +582,9 @@ def main():
         result["gamma_fp32_vs_fp64"] = {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max",
                                         "max_rel_dose_diff_above_cutoff":
                                             float(np.max(np.abs(dose[m] - d64[m]) / d64.max())),
-                                        "fp64_histories_per_s": nhist / st64["seconds"]}
+                                        "fp64_histories_per_s": nhist / st64["seconds"],
+                                        "voxel_error": (voxel_error_report(dose, sum2, d64, args.nbatch)
+                                                        if args.nbatch > 1 else None)}
     if args.beamlets > 0:
         result["beamlets"] = beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, args.beamlets,
                                          sched=None if not fixed else sched, plain=not args.no_plain)
@@ -489,6 +596,10 @@ def main():
         ph.close()
         t.close()
         tables3 = __graft_entry__.tables_dir(3)
+        try:
+            result["config1"] = config1_leg(dpm_amd, gamma, tables3, cpu=not args.no_cpu)
+        except Exception as e:
+            result["config1"] = {"error": "%s: %s" % (type(e).__name__, e)}
         for key, leg in (("config3", config3_leg), ("config5", config5_leg)):
             try:
                 result[key] = leg(dpm_amd, gamma, phantoms, tables3, cpu=not args.no_cpu,

@@ -322,6 +322,9 @@ __global__ void k_round_prep(int* ctl, int nxt) {
 
 // histories per queue fill when the caller does not choose
 constexpr int64_t kDefaultChunk = (int64_t)1 << 24;
+// smallest chunk the autotuner times (below it the fixed cost per chunk --
+// drain rounds, fold, kernel tails -- dominates)
+constexpr int64_t kMinTunedChunk = (int64_t)1 << 21;
 
 // A run's chunks go to kPipes pipes, each with its own stream, queues and
 // control words, that run side by side: a pipe's kernel tails, drain rounds
@@ -823,15 +826,6 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   const int npipes = pipe_count(opt);
   const int wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
   const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
-  // secondaries of a chunk, plus the unfilled tails of the per-wave append
-  // blocks (at most one block per wave and run of an output queue -- the
-  // electron queues have two runs -- and kernel launch)
-  const int64_t holes = (int64_t)3 * blocks * (256 / 64) * 64;
-  const int cap = (int)std::min<int64_t>((int64_t)16 * chunk + holes + 65536, (int64_t)1 << 30);
-  for (int q = 0; q < npipes; ++q) {
-    const int rc = ws[q].ensure(cap, t->ctl + q * kCtlInts);
-    if (rc != DPM_OK) return rc;
-  }
   SourceParams* dS = nullptr;
   int rc = upload_source(t, hs, p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox, nbatch,
                          opt->hist_begin, nhist, &dS);
@@ -868,6 +862,19 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
                              PrimRange{std::max(a, n_ph) - n_ph, std::max(b, n_ph) - n_ph, bfirst}, bfirst, nrows});
       done += n;
     }
+  }
+  // queue capacity: secondaries of the largest chunk (a chunk of whole
+  // batches can exceed `chunk` when one batch does), plus the unfilled tails
+  // of the per-wave append blocks (at most one block per wave and run of an
+  // output queue -- the electron queues have two runs -- and kernel launch)
+  int64_t largest = 1;
+  for (const Chunk& c : chunks)
+    largest = std::max<int64_t>(largest, (int64_t)(c.r_ph.end - c.r_ph.beg) + (int64_t)(c.r_el.end - c.r_el.beg));
+  const int64_t holes = (int64_t)3 * blocks * (256 / 64) * 64;
+  const int cap = (int)std::min<int64_t>((int64_t)16 * largest + holes + 65536, (int64_t)1 << 30);
+  for (int q = 0; q < npipes; ++q) {
+    const int rc = ws[q].ensure(cap, t->ctl + q * kCtlInts);
+    if (rc != DPM_OK) return rc;
   }
 
   int rounds = 0;
@@ -1560,12 +1567,10 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
   if (p->geom == GEOM_SLAB || nbatch < 2 || (uint64_t)nbatch > n) nbatch = 0;
   // the chunks of a real run of the sample (the automatic plan of each pipe count)
   ChunkPlan plan[kPipes + 1];
-  int rows_of[kPipes + 1];
   int rows_max = rows;
   for (int np = 1; np <= kPipes; ++np) {
     plan[np] = plan_chunks(n, nbatch, 0, np);
-    rows_of[np] = nbatch > 0 ? plan[np].rows * np : rows;
-    rows_max = std::max(rows_max, rows_of[np]);
+    rows_max = std::max(rows_max, nbatch > 0 ? plan[np].rows * np : rows);
   }
   t->rows_zero = 0;   // candidates leave their tallies in the rows
   if (!t->buf_rows.ensure(sizeof(double) * nb * rows_max) ||
@@ -1575,39 +1580,51 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
   static const int kWaves[] = {8, 12, 16};
   static const int kRefill[] = {1, 4, 16};
   static const int kTally[] = {0, 1, 2};
+  // device time of one schedule (best of two runs of the sample)
+  auto time_schedule = [&](const dpm_run_options& opt, const ChunkPlan& cp, double* sec) -> int {
+    const int np = pipe_count(&opt);
+    const int rows_run = nbatch > 0 ? cp.rows * np : rows;
+    if (!t->buf_rows.ensure(sizeof(double) * nb * rows_run)) return set_error(DPM_E_NOMEM, "cannot allocate the tally");
+    *sec = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      (void)hipMemsetAsync(t->buf_rows.p, 0, sizeof(double) * nb * rows_run, t->stream);
+      if (p->geom == GEOM_SLAB) (void)hipMemsetAsync(t->buf_steps.p, 0, sizeof(double) * nb, t->stream);
+      (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream);
+      dpm_stats st;
+      std::memset(&st, 0, sizeof(st));
+      const int rc = run_transport<R>(t, p, hs, &opt, ws, T, cp, t->buf_rows.d(),
+                                      p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr,
+                                      nbatch > 0 ? t->buf_sum.d() : nullptr, nbatch > 0 ? t->buf_sum2.d() : nullptr,
+                                      nullptr, &st);
+      if (rc != DPM_OK) return rc;
+      *sec = std::min(*sec, st.seconds);
+    }
+    return DPM_OK;
+  };
   dpm_schedule best;
   std::memset(&best, 0, sizeof(best));
   double best_s = 1e300, default_s = 0.0;
   int cand = 0;
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  opt.seed = 0x5eed;
+  opt.hist_begin = 0;
+  opt.nbatch = nbatch;
+  // stage 1: pipes x waves per CU x refill threshold x tally mode, each with
+  // the automatic chunk plan of its pipe count
   for (int np = 1; np <= kPipes; ++np)
     for (int w : kWaves)
       for (int r : kRefill)
         for (int tm : kTally) {
           if (tm != 0 && p->geom == GEOM_SLAB) continue;  // the slab tally is privatised in LDS either way
-          dpm_run_options opt;
-          std::memset(&opt, 0, sizeof(opt));
-          opt.seed = 0x5eed;
-          opt.hist_begin = 0;
           opt.chunk = 0;
-          opt.nbatch = nbatch;
           opt.waves_per_cu = w;
           opt.refill_min = r;
           opt.tally_mode = tm;
           opt.pipes = np;
-          double sec = 1e300;
-          for (int rep = 0; rep < 2; ++rep) {
-            (void)hipMemsetAsync(t->buf_rows.p, 0, sizeof(double) * nb * rows_of[np], t->stream);
-            if (p->geom == GEOM_SLAB) (void)hipMemsetAsync(t->buf_steps.p, 0, sizeof(double) * nb, t->stream);
-            (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), t->stream);
-            dpm_stats st;
-            std::memset(&st, 0, sizeof(st));
-            const int rc = run_transport<R>(t, p, hs, &opt, ws, T, plan[np], t->buf_rows.d(),
-                                            p->geom == GEOM_SLAB ? t->buf_steps.d() : nullptr,
-                                            nbatch > 0 ? t->buf_sum.d() : nullptr,
-                                            nbatch > 0 ? t->buf_sum2.d() : nullptr, nullptr, &st);
-            if (rc != DPM_OK) return rc;
-            sec = std::min(sec, st.seconds);
-          }
+          double sec;
+          const int rc = time_schedule(opt, plan[np], &sec);
+          if (rc != DPM_OK) return rc;
           ++cand;
           if (np == 1 && w == 16 && r == 1 && tm == 0) default_s = sec;
           if (sec < best_s) {
@@ -1618,7 +1635,27 @@ int autotune_impl(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, int
             best.pipes = np;
           }
         }
-  best.chunk = 0;   // the automatic chunk plan of the chosen pipe count, as timed
+  // stage 2: the chunk (histories per queue fill) at the best schedule --
+  // caps of 2^21..2^24 histories that split the sample into two or more
+  // chunks, each holding at least one whole batch; 0 keeps the automatic plan
+  best.chunk = 0;
+  opt.waves_per_cu = best.waves_per_cu;
+  opt.refill_min = best.refill_min;
+  opt.tally_mode = best.tally_mode;
+  opt.pipes = best.pipes;
+  for (int64_t c = kMinTunedChunk; c <= kDefaultChunk; c <<= 1) {
+    if ((uint64_t)c >= n || c == plan[best.pipes].chunk) continue;
+    if (nbatch > 0 && (uint64_t)c < (n + (uint64_t)nbatch - 1) / (uint64_t)nbatch) continue;
+    opt.chunk = c;
+    double sec;
+    const int rc = time_schedule(opt, plan_chunks(n, nbatch, c, best.pipes), &sec);
+    if (rc != DPM_OK) return rc;
+    ++cand;
+    if (sec < 0.98 * best_s) {   // a fixed chunk must win by more than the timing noise
+      best_s = sec;
+      best.chunk = c;
+    }
+  }
   best.candidates = cand;
   best.histories_per_s = (double)n / best_s;
   best.default_histories_per_s = default_s > 0 ? (double)n / default_s : 0.0;

@@ -512,7 +512,8 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   t.rng.hist = (uint32_t)hid;
   t.rng.slo = 0; t.rng.shi = 0;
   t.rng.ndraw = 0; t.rng.nchild = 0; t.rng.nblk = 0;
-  t.rng.topup();
+  // no top-up here: the first draw fills a block, and the transport loop
+  // tops the buffer up at its next wave-uniform point
   const double x0 = L.x[lo], y0 = L.y[lo];
   t.pos[0] = (R)x0;
   t.pos[1] = (R)y0;
@@ -723,9 +724,12 @@ struct InRange {
   int next, end;
 };
 
+// The indices are handed out first (over as many ranges as it takes) and the
+// start code then runs once for all refilled lanes.
 template <class Start>
 __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, int* head, int n_in, Start start) {
   uint64_t need = __ballot(!has);
+  int idx = -1;
   while (need != 0ull && !exhausted) {
     if (r.next >= r.end) {
       int b = 0;
@@ -739,11 +743,11 @@ __device__ __forceinline__ void refill(bool& has, bool& exhausted, InRange& r, i
     const bool mine = ((need >> lane_id()) & 1ull) != 0ull;
     const int rank = (int)__popcll(need & lanes_below());
     const bool take = mine && rank < avail;
-    const int idx = r.next + rank;
+    if (take) idx = r.next + rank;
     r.next += min(avail, (int)__popcll(need));
-    if (take) has = start(idx);
     need &= ~__ballot(take);
   }
+  if (idx >= 0) has = start(idx);
 }
 
 template <int GEOM, int LT>
@@ -787,7 +791,8 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
           escape(acc, t.E);
           return false;
         }
-        t.rng.topup();
+        // the buffer is filled by the step's top-up (one pass over the
+        // wave for new and running lanes alike)
         return start_track<GEOM>(acc, P, t);
       });
     if (!__any(has)) {
@@ -891,7 +896,16 @@ __device__ __forceinline__ void load_lds_splines(const Tables<R>& T, LdsSplines&
 template <int GEOM, bool LDS, class BlockAcc>
 __device__ __forceinline__ bool electron_init(BlockAcc& a, const KParams<R>& P, const ESplines<LDS>& SP, Track& t,
                                               EState& s) {
-  if (!start_track<GEOM>(a, P, t)) return false;
+  if constexpr (GEOM == GEOM_VOXEL) {
+    // The DistanceToBoundary call at a track's start (SimDPMLike.cc:86-95)
+    // only pushes a position lying on a face and tests the phantom's bounds;
+    // the step's own call (SimDPMLike.cc:257), made next on the same
+    // position, does the same push and the same test (escape with the
+    // track's energy), so in a voxel phantom it is left to the step.
+    if (t.type == (int)kEmptySlot) return false;
+  } else if (!start_track<GEOM>(a, P, t)) {
+    return false;
+  }
   s.hinge = true;
   s.E0 = t.E;
   s.nTr1 = SP.k1(t.E);
@@ -1154,7 +1168,8 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
           escape(acc, t.E);
           return false;
         }
-        t.rng.topup();
+        // electron_init's first draw fills one block; the step's top-up
+        // adds the second
         return electron_init<GEOM, LDS>(acc, P, SP, t, s);
       });
     if (!__any(has)) {

@@ -262,13 +262,16 @@ int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val);
 
 /* ---------------------------------------------------------------------------
  * Autotuner (no counterpart in the reference, whose transport has no launch
- * schedule): times candidate schedules of the persistent kernels -- waves per
- * CU, refill threshold (compaction interval) and voxel-tally mode -- on a
- * sample of `nsample` histories of the given source (or beamlets, scaled to
- * about nsample histories in total) and returns the fastest. The chunk is
- * not sampled: candidates run with the automatic chunk plan of their pipe
- * count, and `chunk` is returned as 0 (automatic). Copy the fields into
- * dpm_run_options for the real run.
+ * schedule): times candidate schedules of the persistent kernels -- pipes,
+ * waves per CU, refill threshold (compaction interval) and voxel-tally mode,
+ * each with the automatic chunk plan of its pipe count; then, at the best of
+ * those, the queue chunk size (caps of 2^21..2^24 histories that leave at
+ * least two chunks of whole batches) -- on a sample of `nsample` histories of
+ * the given source (or beamlets, scaled to about nsample histories in total)
+ * and returns the fastest. `chunk` is 0 when the automatic plan won (a fixed
+ * chunk must be faster by 2 %). Copy the fields into dpm_run_options for the
+ * real run; sample with the real run's batch size so that a chunk holds the
+ * same whole batches in both.
  * ------------------------------------------------------------------------- */
 typedef struct dpm_schedule {
   int64_t chunk;

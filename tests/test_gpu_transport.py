@@ -189,9 +189,17 @@ def test_autotune_returns_a_timed_candidate(gpu_tables0):
     ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
     s = dpm_amd.autotune(gpu_tables0, ph, nsample=20000, particle=0, emin=0.5, emax=20.0, field=40.0)
     assert s["waves_per_cu"] in (8, 12, 16) and s["refill_min"] in (1, 4, 16) and s["tally_mode"] in (0, 1, 2)
-    # the candidates ran with the automatic chunk plan (chunk 0) on one and
-    # on two pipes
+    # 54 schedules with the automatic chunk plan on one and on two pipes; a
+    # sample below 2^21 histories leaves no chunk size to try (chunk 0)
     assert s["candidates"] == 54 and s["chunk"] == 0 and s["pipes"] in (1, 2)
+    # a sample of 4.2e6 histories: chunks of 2^21 and 2^22 histories are
+    # timed at the best schedule, and the one returned runs
+    n4 = 4_200_000
+    s4 = dpm_amd.autotune(gpu_tables0, ph, nsample=n4, particle=0, emin=0.5, emax=20.0, field=40.0, nbatch=4)
+    assert s4["candidates"] == 56 and s4["chunk"] in (0, 1 << 21, 1 << 22)
+    d4, _, st4, _ = dpm_amd.simulate(gpu_tables0, ph, n4, particle=0, emin=0.5, emax=20.0, field=40.0, nbatch=4,
+                                     schedule=s4)
+    assert d4.sum() > 0 and st4["e_primary"] > 0
     assert s["histories_per_s"] >= s["default_histories_per_s"] > 0
     bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000)]
     sb = dpm_amd.autotune_beamlets(gpu_tables0, ph, bl, nsample=5000)
