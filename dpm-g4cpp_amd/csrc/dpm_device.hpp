@@ -490,6 +490,51 @@ __device__ __forceinline__ R distance_to_boundary(const Geometry<R>& g, R r[3], 
   return (R)-1.0;   // pathological: treat as leaving the geometry
 }
 
+// DistanceToBoundary where only the box, the tolerance push and the extent
+// test are used (Woodcock photon steps, track starts, the stopping
+// positron's position): the loop of distance_to_boundary -- extent test and
+// box, then the first axis in x, y, z order whose face ahead lies within half
+// the tolerance pushes the position by the tolerance and the loop re-enters --
+// without the face distances. False where distance_to_boundary returns -1.
+template <typename R, int GEOM>
+__device__ __forceinline__ bool locate(const Geometry<R>& g, R r[3], const R v[3], int box[3]) {
+  const R kHalfTolerance = (R)(0.5 * kTolerance);
+  for (int it = 0; it < 16; ++it) {
+    R tr[3];
+    if (GEOM == GEOM_SLAB) {
+      if (rabs(r[0]) > (R)kExtent || rabs(r[1]) > (R)kExtent || r[2] > (R)kExtent || r[2] < -g.L) return false;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        box[a] = (int)rfloor((r[a] + g.hL) * g.invL);
+        tr[a] = r[a] - (R)box[a] * g.L;
+      }
+    } else {
+      const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const R uu = r[a] - g.org[a];
+        const R fl = rfloor(uu * g.invL);
+        if (!(fl >= (R)0.0) || fl >= (R)n[a]) return false;
+        box[a] = (int)fl;
+        tr[a] = uu - ((R)box[a] + (R)0.5) * g.L;
+      }
+    }
+    int push = -1;
+    R sgn = (R)0;
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {   // the lowest qualifying axis wins, as in local_distance
+      if (v[a] > (R)0.0 && g.hL - tr[a] < kHalfTolerance) { push = a; sgn = (R)1; }
+      else if (v[a] < (R)0.0 && g.hL + tr[a] < kHalfTolerance) { push = a; sgn = (R)-1; }
+    }
+    if (push < 0) return true;
+    const R d = sgn * (R)kTolerance;
+    if (push == 0) r[0] += d;
+    else if (push == 1) r[1] += d;
+    else r[2] += d;
+  }
+  return false;   // pathological: treat as leaving the geometry
+}
+
 // box of a position without the extent check (the reference keeps the
 // parent's box when DistanceToBoundary returns -1 at a track's start)
 template <typename R, int GEOM>

@@ -622,7 +622,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, co
   t.pos[0] += t.dir[0] * stepLength;
   t.pos[1] += t.dir[1] * stepLength;
   t.pos[2] += t.dir[2] * stepLength;
-  if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) { escape(a, t.E); return false; }
+  if (!locate<R, GEOM>(P.G, t.pos, t.dir, t.box)) { escape(a, t.E); return false; }
   count(a, kCntVoxelVisits);
   const int imat = material_index<R, GEOM>(P.G, t.box);
   if (imat < 0) { escape(a, t.E); return false; }
@@ -700,7 +700,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, co
 template <int GEOM, class BlockAcc>
 __device__ __forceinline__ bool start_track(BlockAcc& a, const KParams<R>& P, Track& t) {
   if (t.type == (int)kEmptySlot) return false;
-  if (distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box) < (R)0) box_of<R, GEOM>(P.G, t.pos, t.box);
+  if (!locate<R, GEOM>(P.G, t.pos, t.dir, t.box)) box_of<R, GEOM>(P.G, t.pos, t.box);
   if constexpr (GEOM == GEOM_VOXEL) {
     // every voxel holds a valid material (checked when the phantom was
     // created), so the material lookup fails exactly outside the phantom:
@@ -967,7 +967,7 @@ __device__ __forceinline__ bool electron_step(BlockAcc& a, const KParams<R>& P, 
       t.pos[0] += t.dir[0] * stepLength;
       t.pos[1] += t.dir[1] * stepLength;
       t.pos[2] += t.dir[2] * stepLength;
-      distance_to_boundary<R, GEOM>(P.G, t.pos, t.dir, t.box);
+      locate<R, GEOM>(P.G, t.pos, t.dir, t.box);
       count(a, kCntVoxelVisits);
       imat = material_index<R, GEOM>(P.G, t.box);
       if (imat < 0) { escape(a, edep); return false; }
