@@ -263,6 +263,18 @@ def config1_leg(dpm_amd, gamma, tables3, cpu=True, n=100_000):
         multi = time.perf_counter() - t0
         if any(rcs):
             raise RuntimeError("reference copies failed: %s" % rcs)
+        # the port (the oracle used as the CPU baseline of configurations 2, 3
+        # and 5, where the reference cannot run) on this same workload in its
+        # mt19937 mode, which reproduces the reference's histories: both
+        # single-threaded, wall time of the process incl. table loading
+        port_s = None
+        port_bin = os.path.join(ROOT, "oracle", "build", "dpm_oracle_cli")
+        if os.path.exists(port_bin):
+            t0 = time.perf_counter()
+            subprocess.check_call([port_bin, "--data", tables3, "--mode", "mt", "--seed", "123", "--geom", "slab",
+                                   "--lbox", "1.0", "--gidx", "3", "--ptype", "-1", "--e0", "15", "--n", str(n)],
+                                  stdout=subprocess.DEVNULL)
+            port_s = time.perf_counter() - t0
         mg, hg = _read_hist_sim(gpu_out)
         mr, hr = _read_hist_sim(ref_out)
         m = min(len(hg), len(hr))
@@ -271,6 +283,9 @@ def config1_leg(dpm_amd, gamma, tables3, cpu=True, n=100_000):
                              "sample": "%d concurrent copies of the reference dpm_Simulate (compiled from its "
                                        "sources, -O3), each the full %d histories, wall %.1f s incl. table loading; "
                                        "single copy %.2f s = %.0f hist/s" % (cores, n, multi, single, n / single)}
+        if port_s is not None:
+            r["cpu_baseline"]["port_single_core_histories_per_s"] = n / port_s
+            r["cpu_baseline"]["port_vs_reference_single_core"] = single / port_s
         r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
         r["equal_history_depth_dose"] = {
             "gamma_pass_rate_gpu_fp32_vs_reference": pr,
