@@ -53,7 +53,7 @@ Geant4):
   tabulated for rational-interpolation (RITA) inversion.
 
 The tables are therefore physically plausible but not Geant4's; GPU-vs-CPU
-parity is judged on identical files written by this module.
+parity is checked on identical files written by this module.
 """
 import argparse
 import math

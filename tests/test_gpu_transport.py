@@ -5,7 +5,7 @@
    history -- only transcendental-function last-bit differences between the
    GPU and glibc can make a history take another branch.
 2. FP32 GPU transport vs the oracle at equal history count: 1 %/1 mm gamma
-   pass rate >= 99 % over bins above 10 % of the maximum.
+   pass rate >= 99 % over bins above 10 % of the maximum (the acceptance bar).
 3. Energy conservation per run: scored + escaped = primaries + 2 mc^2
    (annihilations - pairs), to 1e-9 (FP64) / 1e-5 (FP32).
 """
