@@ -1242,6 +1242,21 @@ double oracle_eval(int which, double e, int imat) {
 
 void oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox10(ctr, key, out); }
 
+// Geometry::distance on a standalone geometry: the reference slab
+// (Geom.cc:20-117, boxes of size lbox centred on i*lbox, extent +-100 mm) or a
+// centred nx*ny*nz voxel phantom; r is updated by the tolerance push
+double oracle_distance(int geom_mode, double lbox, int nx, int ny, int nz, double* r, const double* v, int* box) {
+  Geometry geo;
+  geo.mode = geom_mode;
+  geo.L = lbox; geo.invL = 1. / lbox; geo.hL = 0.5 * lbox;
+  geo.nx = nx; geo.ny = ny; geo.nz = nz;
+  geo.org[0] = -0.5 * nx * lbox; geo.org[1] = -0.5 * ny * lbox; geo.org[2] = -0.5 * nz * lbox;
+  return geo.distance(r, v, box);
+}
+
+// RotateToLabFrame (SimDPMLike.cc:638-656) on uvw[3] with the frame's axis (u1, u2, u3)
+void oracle_rotate(double* uvw, double u1, double u2, double u3) { Sim::rotate(uvw[0], uvw[1], uvw[2], u1, u2, u3); }
+
 void oracle_set_fault(int f) { g_fault = f; }
 
 }  // extern "C"

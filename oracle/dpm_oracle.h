@@ -92,6 +92,12 @@ double oracle_eval(int which, double e, int imat);
 void   oracle_set_fault(int f);
 /* Philox4x32-10 block for tests */
 void   oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* geometry probe: distance to the next voxel face from r along v (r is
+ * pushed by the tolerance rule), box of r; -1 when r is outside. Slab: the
+ * reference Geom with boxes of lbox; voxel: a centred nx*ny*nz phantom */
+double oracle_distance(int geom_mode, double lbox, int nx, int ny, int nz, double* r, const double* v, int* box);
+/* RotateToLabFrame: uvw (in the frame of axis u) -> lab frame, in place */
+void   oracle_rotate(double* uvw, double u1, double u2, double u3);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,11 @@ def lib():
         L.oracle_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.oracle_philox.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_distance.restype = ctypes.c_double
+        L.oracle_distance.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.oracle_rotate.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -181,6 +186,22 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     lib().oracle_philox(c, k, o)
     return list(o)
+
+
+def distance(geom_mode, lbox, r, v, n=(0, 0, 0)):
+    """The oracle's DistanceToBoundary: (distance or -1, box, position after the tolerance push)."""
+    rr = (ctypes.c_double * 3)(*r)
+    vv = (ctypes.c_double * 3)(*v)
+    bb = (ctypes.c_int * 3)()
+    d = lib().oracle_distance(int(geom_mode), float(lbox), int(n[0]), int(n[1]), int(n[2]), rr, vv, bb)
+    return d, tuple(bb), tuple(rr)
+
+
+def rotate(uvw, axis):
+    """The oracle's RotateToLabFrame: the local vector uvw in the frame of `axis`, in the lab frame."""
+    a = (ctypes.c_double * 3)(*uvw)
+    lib().oracle_rotate(a, float(axis[0]), float(axis[1]), float(axis[2]))
+    return tuple(a)
 
 
 def slab_layer_material(iz, lbox, geom_index):
