@@ -157,7 +157,9 @@ typedef struct dpm_run_options {
   int precision;                   /* DPM_FP32 (default) or DPM_FP64 */
   int nbatch;                      /* >1: per-batch sum and sum of squares of the voxel tally */
   int64_t chunk;                   /* histories per queue fill; 0 = automatic (one pipe: up to 16M
-                                      histories; two pipes: an even number of chunks of up to 8M) */
+                                      histories; two pipes: an even number of chunks of up to 8M).
+                                      With nbatch > 1 a chunk holds whole batches, so it exceeds
+                                      this when one batch does (queues are sized for it) */
   int waves_per_cu;                /* persistent waves per CU; 0 = automatic */
   double* hist_edep;               /* optional [nhist] per-history scored energy (host) */
   int refill_min;                  /* idle lanes of a wave that trigger its refill (1..64); 0 = 1 */
