@@ -662,6 +662,7 @@ struct HostSource {
   double emin = 0, emax = 0, fx = 0, fy = 0;
   int spectrum = DPM_SPEC_FLAT;
   double emode = 0;
+  std::vector<double> spec_edges, spec_cdf;   // DPM_SPEC_TABLE
   std::vector<HostEntry> list[2];   // [0] photons, [1] e-/e+
   void add(const HostEntry& h) { list[h.particle == DPM_PHOTON ? 0 : 1].push_back(h); }
   uint64_t count(int k) const {
@@ -692,6 +693,11 @@ void gantry_cos_sin(double deg, double* c, double* s) {
 int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, uint64_t hist_begin, uint64_t nhist,
                   SourceParams** out) {
   size_t bytes = (sizeof(SourceParams) + 255) & ~(size_t)255;
+  const size_t nsp = hs.spec_edges.size();   // table spectrum: edges and cumulative weights
+  const size_t off_edges = bytes;
+  bytes += (8 * nsp + 255) & ~(size_t)255;
+  const size_t off_cdf = bytes;
+  bytes += (8 * nsp + 255) & ~(size_t)255;
   size_t off[2][9];
   for (int k = 0; k < 2; ++k) {
     const size_t n = hs.list[k].size();
@@ -712,6 +718,13 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   S.emax = hs.emax;
   S.spectrum = hs.spectrum;
   S.emode = hs.emode;
+  if (nsp > 1) {
+    S.spec_n = (int)nsp - 1;
+    std::memcpy(h.data() + off_edges, hs.spec_edges.data(), 8 * nsp);
+    std::memcpy(h.data() + off_cdf, hs.spec_cdf.data(), 8 * nsp);
+    S.spec_edges = (const double*)(base + off_edges);
+    S.spec_cdf = (const double*)(base + off_cdf);
+  }
   S.fx = hs.fx;
   S.fy = hs.fy;
   S.z0 = z0;
@@ -1095,6 +1108,16 @@ HostSource source_from(const dpm_source* src, uint64_t nhist, uint64_t hist_begi
   hs.fy = src->field_y;
   hs.spectrum = src->spectrum;
   hs.emode = src->emode;
+  if (src->spectrum == DPM_SPEC_TABLE) {
+    // cumulative weights, normalised by their total; the last is 1 exactly
+    const int n = src->spec_n;
+    hs.spec_edges.assign(src->spec_edges, src->spec_edges + n + 1);
+    hs.spec_cdf.assign(n + 1, 0.0);
+    for (int i = 0; i < n; ++i) hs.spec_cdf[i + 1] = hs.spec_cdf[i] + src->spec_weights[i];
+    const double total = hs.spec_cdf[n];
+    for (int i = 1; i < n; ++i) hs.spec_cdf[i] /= total;
+    hs.spec_cdf[n] = 1.0;
+  }
   HostEntry h{hist_begin, nhist, src->x0, src->y0, src->energy, src->particle, -1};
   h.angle_deg = src->angle_deg;
   hs.add(h);
@@ -1110,8 +1133,22 @@ int validate_source(const dpm_tables* t, const dpm_phantom* p, const dpm_source*
   if (src->kind == DPM_SRC_GANTRY && p->geom != GEOM_VOXEL)
     return set_error(DPM_E_INVALID, "gantry fields need a voxel phantom");
   if (!std::isfinite(src->angle_deg)) return set_error(DPM_E_INVALID, "gantry angle is not finite");
-  if (src->spectrum != DPM_SPEC_FLAT && src->spectrum != DPM_SPEC_TRIANGULAR)
+  if (src->spectrum != DPM_SPEC_FLAT && src->spectrum != DPM_SPEC_TRIANGULAR && src->spectrum != DPM_SPEC_TABLE)
     return set_error(DPM_E_INVALID, "unknown spectrum");
+  if (src->spectrum == DPM_SPEC_TABLE) {
+    if (src->spec_n < 1 || src->spec_n > (1 << 20) || !src->spec_edges || !src->spec_weights)
+      return set_error(DPM_E_INVALID, "spectrum table needs 1 .. 2^20 bins with edges and weights");
+    double total = 0.0;
+    for (int i = 0; i < src->spec_n; ++i) {
+      const double w = src->spec_weights[i];
+      if (!std::isfinite(w) || w < 0.0) return set_error(DPM_E_INVALID, "spectrum weights must be finite and >= 0");
+      if (!(src->spec_edges[i + 1] > src->spec_edges[i]) || !std::isfinite(src->spec_edges[i + 1]))
+        return set_error(DPM_E_INVALID, "spectrum bin edges must increase");
+      total += w;
+    }
+    if (!(total > 0.0) || !std::isfinite(total)) return set_error(DPM_E_INVALID, "spectrum weights sum to zero");
+    return check_source(t, src->particle, src->energy, src->spec_edges[0], src->spec_edges[src->spec_n]);
+  }
   if (src->spectrum == DPM_SPEC_TRIANGULAR && !(src->emax > src->emin && src->emode >= src->emin &&
                                                 src->emode <= src->emax))
     return set_error(DPM_E_INVALID, "triangular spectrum needs emin <= emode <= emax, emin < emax");

@@ -111,8 +111,11 @@ struct SourceParams {
   int kind;                 // 0 pencil, 1 rectangular field around the entry point, 2 gantry field
   int nbatch;               // > 1: tally row = batch index - first batch of the chunk
   double emin, emax;        // emax > emin: energy from the spectrum over [emin, emax)
-  int spectrum;             // 0 flat, 1 triangular with mode emode
+  int spectrum;             // 0 flat, 1 triangular with mode emode, 2 table
   double emode;
+  int spec_n;               // table: bins, their edges [spec_n + 1] and the
+  const double* spec_edges; // normalised cumulative weights [spec_n + 1]
+  const double* spec_cdf;   // (0 ... 1)
   double fx, fy, z0;
   uint64_t hist_begin, nhist_total;
   PrimList list[2];         // [0] photons, [1] electrons and positrons

@@ -495,6 +495,23 @@ __device__ __forceinline__ R triangular(R u, R lo, R mode, R hi) {
 
 // returns false when a gantry ray misses the phantom (t.E then holds the
 // primary's energy, which the caller counts as escaped)
+// energy from a tabulated spectrum (histogram bins of relative weight):
+// the bin whose cumulative-weight interval [cdf[i], cdf[i+1]) holds the
+// uniform u = k 2^-24, then the same fraction of the bin's width (one uniform,
+// in FP64 for every precision)
+__device__ __forceinline__ double spectrum_table(int n, const double* __restrict__ edges,
+                                                 const double* __restrict__ cdf, uint32_t k) {
+  const double u = (double)k * (1.0 / 16777216.0);
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  const double f = (u - cdf[lo]) / (cdf[lo + 1] - cdf[lo]);
+  return edges[lo] + f * (edges[lo + 1] - edges[lo]);
+}
+
 __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
                                             const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
                                             double& eprim, int k) {
@@ -524,7 +541,9 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
     t.pos[1] = (R)y0 + (uy - (R)0.5) * (R)S->fy;
   }
   R E = (R)L.e[lo];
-  if (S->emax > S->emin)
+  if (S->spectrum == 2)
+    E = (R)spectrum_table(S->spec_n, S->spec_edges, S->spec_cdf, t.rng.draw24());
+  else if (S->emax > S->emin)
     E = S->spectrum == 1 ? triangular(t.rng.uni<R>(), (R)S->emin, (R)S->emode, (R)S->emax)
                          : (R)S->emin + t.rng.uni<R>() * (R)(S->emax - S->emin);
   eprim += (double)E;

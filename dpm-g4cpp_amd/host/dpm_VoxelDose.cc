@@ -10,6 +10,7 @@
 //   -p particle (e-, gamma, e+) -e energy [MeV]  or  -E emin,emax (uniform spectrum)
 //   -F field side [mm] (0: pencil)  -X -Y entry point [mm]
 //   -A gantry angle [deg] (field through the phantom centre)  -T mode of a triangular spectrum (with -E)
+//   -S spectrum file (lines "E_low E_high weight" [MeV], contiguous bins)
 //   -n histories  -k batches  -s seed  -f FP64 transport  -g device  -o output
 //   -B beamlet list (lines "x y particle energy histories"; writes CSR)  -t CSR threshold
 #include <getopt.h>
@@ -31,7 +32,7 @@
 namespace {
 
 struct Args {
-  std::string data = "./data", mat_file, rho_file, beamlets, out = "dose.bin", particle = "gamma";
+  std::string data = "./data", mat_file, rho_file, beamlets, out = "dose.bin", particle = "gamma", spectrum;
   int nx = 0, ny = 0, nz = 0, nbatch = 10, device = 0;
   double lbox = 2.0, energy = 10.0, emin = 0.0, emax = 0.0, field = 0.0, x0 = 0.0, y0 = 0.0, threshold = 1e-3;
   double angle = 0.0, emode = 0.0;
@@ -54,6 +55,7 @@ void Help() {
                "  -X -Y mm   entry point                              (default 0 0)\n"
                "  -A deg     gantry angle: beam (sin a, 0, cos a) through the phantom centre\n"
                "  -T MeV     mode of a triangular spectrum over -E lo,hi\n"
+               "  -S file    spectrum file: lines \"E_low E_high weight\" (MeV), contiguous bins\n"
                "  -n N       histories                                (default 1e5)\n"
                "  -k K       history batches for the uncertainty      (default 10)\n"
                "  -s seed    Philox seed                              (default 1)\n"
@@ -93,10 +95,39 @@ void Check(int rc, const char* what) {
 
 }  // namespace
 
+// spectrum file: one bin per line, "E_low E_high weight" in MeV, each bin
+// starting where the previous ends; '#' starts a comment line
+static bool ReadSpectrum(const std::string& path, std::vector<double>& edges, std::vector<double>& weights) {
+  std::ifstream f(path);
+  if (!f) {
+    std::cerr << " *** cannot read the spectrum file " << path << std::endl;
+    return false;
+  }
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream is(line);
+    double lo, hi, w;
+    if (!(is >> lo >> hi >> w)) continue;
+    if (edges.empty()) edges.push_back(lo);
+    else if (lo != edges.back()) {
+      std::cerr << " *** spectrum bins must be contiguous (" << path << ": " << lo << ")" << std::endl;
+      return false;
+    }
+    edges.push_back(hi);
+    weights.push_back(w);
+  }
+  if (weights.empty()) {
+    std::cerr << " *** no bins in the spectrum file " << path << std::endl;
+    return false;
+  }
+  return true;
+}
+
 int main(int argc, char* argv[]) {
   Args a;
   int c;
-  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:n:k:s:fg:B:t:o:")) != -1) {
+  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:S:n:k:s:fg:B:t:o:")) != -1) {
     switch (c) {
       case 'd': a.data = optarg; break;
       case 'x': a.nx = std::atoi(optarg); break;
@@ -119,6 +150,7 @@ int main(int argc, char* argv[]) {
       case 'Y': a.y0 = std::atof(optarg); break;
       case 'A': a.angle = std::atof(optarg); a.gantry = true; break;
       case 'T': a.emode = std::atof(optarg); a.triangular = true; break;
+      case 'S': a.spectrum = optarg; break;
       case 'n': a.nhist = (uint64_t)std::atof(optarg); break;
       case 'k': a.nbatch = std::atoi(optarg); break;
       case 's': a.seed = (uint64_t)std::atoll(optarg); break;
@@ -207,6 +239,14 @@ int main(int argc, char* argv[]) {
     src.angle_deg = a.angle;
     src.spectrum = a.triangular ? DPM_SPEC_TRIANGULAR : DPM_SPEC_FLAT;
     src.emode = a.emode;
+    std::vector<double> sp_edges, sp_weights;
+    if (!a.spectrum.empty()) {
+      if (!ReadSpectrum(a.spectrum, sp_edges, sp_weights)) return EXIT_FAILURE;
+      src.spectrum = DPM_SPEC_TABLE;
+      src.spec_n = (int)sp_weights.size();
+      src.spec_edges = sp_edges.data();
+      src.spec_weights = sp_weights.data();
+    }
     src.particle = ParticleCode(a.particle);
     src.energy = a.energy;
     src.emin = a.emin;

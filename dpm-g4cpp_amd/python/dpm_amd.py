@@ -18,7 +18,7 @@ VOXEL_CLI_PATH = os.path.join(PKG, "bin", "dpm_VoxelDose")
 DPM_FP32, DPM_FP64 = 0, 1
 ELECTRON, PHOTON, POSITRON = -1, 0, 1
 SRC_PENCIL, SRC_FIELD, SRC_GANTRY = 0, 1, 2
-SPEC_FLAT, SPEC_TRIANGULAR = 0, 1
+SPEC_FLAT, SPEC_TRIANGULAR, SPEC_TABLE = 0, 1, 2
 COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries", "e_steps", "g_steps",
                  "tally_atomics")
 
@@ -37,7 +37,8 @@ class Source(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("particle", ctypes.c_int), ("energy", ctypes.c_double),
                 ("emin", ctypes.c_double), ("emax", ctypes.c_double), ("x0", ctypes.c_double),
                 ("y0", ctypes.c_double), ("field_x", ctypes.c_double), ("field_y", ctypes.c_double),
-                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double)]
+                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double),
+                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p)]
 
 
 class Field(ctypes.Structure):
@@ -259,7 +260,7 @@ def pinned_empty(n, dtype=np.float64):
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
              out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
-             angle=None, emode=None, table_memory=0, pipes=0):
+             angle=None, emode=None, table_memory=0, pipes=0, spectrum_table=None):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -268,7 +269,7 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     device (tally and steps are returned as None; see tally_fetch). angle
     (degrees) selects the gantry source (voxel phantoms), emode the triangular
     spectrum (emin, emode, emax)."""
-    src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode)
+    src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode, spectrum_table)
     he = np.zeros(nhist) if per_history else None
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
@@ -299,11 +300,22 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     return tally, steps, st.as_dict(), he
 
 
-def _source(particle, energy, emin, emax, field, x0, y0, angle, emode):
+def _source(particle, energy, emin, emax, field, x0, y0, angle, emode, table=None):
     kind = SRC_GANTRY if angle is not None else (SRC_FIELD if field > 0 else SRC_PENCIL)
-    return Source(kind=kind, particle=particle, energy=energy, emin=emin, emax=emax, x0=x0, y0=y0, field_x=field,
+    src = Source(kind=kind, particle=particle, energy=energy, emin=emin, emax=emax, x0=x0, y0=y0, field_x=field,
                   field_y=field, angle_deg=0.0 if angle is None else angle,
                   spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT, emode=0.0 if emode is None else emode)
+    if table is not None:
+        # a spectrum file: histogram bin edges [n + 1] (MeV) and weights [n];
+        # the arrays stay referenced by the structure for the call
+        edges = np.ascontiguousarray(table[0], dtype=np.float64)
+        weights = np.ascontiguousarray(table[1], dtype=np.float64)
+        if edges.size != weights.size + 1:
+            raise ValueError("spectrum table: need len(edges) == len(weights) + 1")
+        src.spectrum, src.spec_n = SPEC_TABLE, weights.size
+        src.spec_edges, src.spec_weights = _ptr(edges), _ptr(weights)
+        src._keep = (edges, weights)
+    return src
 
 
 def simulate_fields(tables, phantom, fields, particle=PHOTON, energy=10.0, emin=0.0, emax=0.0, emode=None,

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DPM_ABI_VERSION 1
+#define DPM_ABI_VERSION 2
 
 enum {
   DPM_OK = 0,
@@ -132,7 +132,7 @@ int64_t dpm_phantom_num_bins(const dpm_phantom* p);
  * Sources and runs
  * ------------------------------------------------------------------------- */
 enum { DPM_SRC_PENCIL = 0, DPM_SRC_FIELD = 1, DPM_SRC_GANTRY = 2 };
-enum { DPM_SPEC_FLAT = 0, DPM_SPEC_TRIANGULAR = 1 };
+enum { DPM_SPEC_FLAT = 0, DPM_SPEC_TRIANGULAR = 1, DPM_SPEC_TABLE = 2 };
 
 typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
   int kind;                        /* DPM_SRC_* */
@@ -147,8 +147,18 @@ typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
      primaries start where their ray enters the phantom. a = 0 is the +z
      field of DPM_SRC_FIELD. */
   double angle_deg;
-  int spectrum;                    /* DPM_SPEC_FLAT, or DPM_SPEC_TRIANGULAR(emin, emode, emax) */
+  int spectrum;                    /* DPM_SPEC_FLAT, or DPM_SPEC_TRIANGULAR(emin, emode, emax), or
+                                      DPM_SPEC_TABLE */
   double emode;
+  /* DPM_SPEC_TABLE (a spectrum file): spec_n histogram bins, bin i covering
+     [spec_edges[i], spec_edges[i+1]) MeV with relative weight
+     spec_weights[i] >= 0 (edges increasing, weights not all zero; emin and
+     emax are not used). A primary's energy takes one uniform u, as the flat
+     spectrum does: the bin whose cumulative weight interval holds u, then the
+     same fraction of that bin's width. The arrays are read during the call. */
+  int spec_n;
+  const double* spec_edges;        /* [spec_n + 1] */
+  const double* spec_weights;      /* [spec_n] */
 } dpm_source;
 
 typedef struct dpm_run_options {

@@ -1047,6 +1047,19 @@ void gantry_cos_sin(double deg, double* c, double* s) {
   *s = std::sin(a);
 }
 
+// energy from a histogram spectrum by inverse CDF with one uniform u: the
+// bin i whose cumulative weight interval [cdf[i], cdf[i+1]) holds u (cdf
+// normalised by the total weight, its last entry 1), then the same fraction
+// of bin i's width
+double table_energy(double u, const std::vector<double>& cdf, const double* edges) {
+  const int n = (int)cdf.size() - 1;
+  int lo = 0;
+  for (int i = 1; i < n; ++i)
+    if (cdf[i] <= u) lo = i;
+  const double f = (u - cdf[lo]) / (cdf[lo + 1] - cdf[lo]);
+  return edges[lo] + f * (edges[lo + 1] - edges[lo]);
+}
+
 // inverse CDF of the triangular distribution (lo, mode, hi)
 double triangular(double u, double lo, double mode, double hi) {
   const double fc = (mode - lo) / (hi - lo);
@@ -1122,6 +1135,19 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
     return 1;
   }
   const double zstart = geo.mode == ORACLE_GEOM_SLAB ? -0.5 * cfg->lbox : geo.org[2];
+  // table spectrum: cumulative weights normalised by the total, last entry 1
+  std::vector<double> spec_cdf;
+  if (cfg->spectrum == ORACLE_SPEC_TABLE) {
+    if (cfg->spec_n < 1 || !cfg->spec_edges || !cfg->spec_weights) {
+      g_err = "spectrum table needs bins, edges and weights";
+      return 1;
+    }
+    spec_cdf.assign(cfg->spec_n + 1, 0.0);
+    for (int i = 0; i < cfg->spec_n; ++i) spec_cdf[i + 1] = spec_cdf[i] + cfg->spec_weights[i];
+    const double total = spec_cdf[cfg->spec_n];
+    for (int i = 1; i < cfg->spec_n; ++i) spec_cdf[i] /= total;
+    spec_cdf[cfg->spec_n] = 1.0;
+  }
   double ca = 1.0, sa = 0.0;
   gantry_cos_sin(cfg->angle_deg, &ca, &sa);
   int curBatch = 0;
@@ -1152,7 +1178,9 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
         p.pos[0] = cfg->x0 + (ux - 0.5) * cfg->fx;
         p.pos[1] = cfg->y0 + (uy - 0.5) * cfg->fy;
       }
-      if (cfg->emax > cfg->emin)
+      if (cfg->spectrum == ORACLE_SPEC_TABLE)
+        p.ekin = table_energy(s.rng.u(), spec_cdf, cfg->spec_edges);
+      else if (cfg->emax > cfg->emin)
         p.ekin = cfg->spectrum == ORACLE_SPEC_TRIANGULAR ? triangular(s.rng.u(), cfg->emin, cfg->emode, cfg->emax)
                                                          : cfg->emin + s.rng.u() * (cfg->emax - cfg->emin);
       ep += p.ekin;
@@ -1252,6 +1280,17 @@ double oracle_distance(int geom_mode, double lbox, int nx, int ny, int nz, doubl
   geo.nx = nx; geo.ny = ny; geo.nz = nz;
   geo.org[0] = -0.5 * nx * lbox; geo.org[1] = -0.5 * ny * lbox; geo.org[2] = -0.5 * nz * lbox;
   return geo.distance(r, v, box);
+}
+
+// the table spectrum's energy for uniforms u[m] (cumulative weights built as
+// for a run)
+void oracle_table_energy(int n, const double* edges, const double* weights, int m, const double* u, double* e) {
+  std::vector<double> cdf(n + 1, 0.0);
+  for (int i = 0; i < n; ++i) cdf[i + 1] = cdf[i] + weights[i];
+  const double total = cdf[n];
+  for (int i = 1; i < n; ++i) cdf[i] /= total;
+  cdf[n] = 1.0;
+  for (int j = 0; j < m; ++j) e[j] = table_energy(u[j], cdf, edges);
 }
 
 // RotateToLabFrame (SimDPMLike.cc:638-656) on uvw[3] with the frame's axis (u1, u2, u3)

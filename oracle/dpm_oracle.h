@@ -27,7 +27,7 @@ extern "C" {
 enum { ORACLE_RNG_MT19937 = 0, ORACLE_RNG_PHILOX = 1 };
 enum { ORACLE_GEOM_SLAB = 0, ORACLE_GEOM_VOXEL = 1 };
 enum { ORACLE_SRC_PENCIL = 0, ORACLE_SRC_FIELD = 1, ORACLE_SRC_GANTRY = 2 };
-enum { ORACLE_SPEC_FLAT = 0, ORACLE_SPEC_TRIANGULAR = 1 };
+enum { ORACLE_SPEC_FLAT = 0, ORACLE_SPEC_TRIANGULAR = 1, ORACLE_SPEC_TABLE = 2 };
 
 typedef struct oracle_cfg {
   int      rng_mode;      /* ORACLE_RNG_* */
@@ -52,8 +52,12 @@ typedef struct oracle_cfg {
      phantom centre, field on the plane through the centre, primaries start
      where their ray enters the phantom */
   double   angle_deg;
-  int      spectrum;      /* ORACLE_SPEC_*: with emax > emin, flat or triangular(emin, emode, emax) */
+  int      spectrum;      /* ORACLE_SPEC_*: with emax > emin, flat or triangular(emin, emode, emax);
+                             table: histogram bins [edges[i], edges[i+1]) of relative weight w[i] */
   double   emode;
+  int      spec_n;
+  const double* spec_edges;   /* [spec_n + 1] */
+  const double* spec_weights; /* [spec_n] */
 } oracle_cfg;
 
 typedef struct oracle_out {
@@ -96,6 +100,8 @@ void   oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
  * pushed by the tolerance rule), box of r; -1 when r is outside. Slab: the
  * reference Geom with boxes of lbox; voxel: a centred nx*ny*nz phantom */
 double oracle_distance(int geom_mode, double lbox, int nx, int ny, int nz, double* r, const double* v, int* box);
+/* table spectrum: energies for uniforms u[m] from n bins (edges[n+1], weights[n]) */
+void   oracle_table_energy(int n, const double* edges, const double* weights, int m, const double* u, double* e);
 /* RotateToLabFrame: uvw (in the frame of axis u) -> lab frame, in place */
 void   oracle_rotate(double* uvw, double u1, double u2, double u3);
 

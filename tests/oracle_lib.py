@@ -15,7 +15,7 @@ CLI_PATH = os.path.join(ROOT, "oracle", "build", "dpm_oracle_cli")
 RNG_MT, RNG_PHILOX = 0, 1
 GEOM_SLAB, GEOM_VOXEL = 0, 1
 SRC_PENCIL, SRC_FIELD, SRC_GANTRY = 0, 1, 2
-SPEC_FLAT, SPEC_TRIANGULAR = 0, 1
+SPEC_FLAT, SPEC_TRIANGULAR, SPEC_TABLE = 0, 1, 2
 
 NCNT = 9
 COUNTER_NAMES = ("voxel_visits", "deposits", "sb", "moller", "kn", "gs", "secondaries",
@@ -32,7 +32,8 @@ class Cfg(ctypes.Structure):
                 ("e0", ctypes.c_double), ("emin", ctypes.c_double), ("emax", ctypes.c_double),
                 ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double),
                 ("hist_begin", ctypes.c_uint64), ("nhist", ctypes.c_uint64), ("nbatch", ctypes.c_int),
-                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double)]
+                ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double),
+                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p)]
 
 
 class Out(ctypes.Structure):
@@ -78,6 +79,8 @@ def lib():
         L.oracle_distance.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int)]
+        L.oracle_table_energy.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_rotate.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
@@ -114,7 +117,8 @@ def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_
 
 
 def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
-              seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None):
+              seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None,
+              spectrum_table=None):
     """Voxel run in Philox mode. angle (degrees) selects the gantry source (a
     field, or a pencil when field == 0, whose ray enters the phantom from the
     side the beam direction (sin a, 0, cos a) points away from); emode selects
@@ -129,6 +133,10 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
               angle_deg=0.0 if angle is None else angle, spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT,
               emode=0.0 if emode is None else emode, ptype=ptype, e0=e0, emin=emin, emax=emax,
               fx=field, fy=field, x0=x0, y0=y0, hist_begin=hist_begin, nhist=nhist, nbatch=nbatch)
+    if spectrum_table is not None:
+        edges = np.ascontiguousarray(spectrum_table[0], dtype=np.float64)
+        weights = np.ascontiguousarray(spectrum_table[1], dtype=np.float64)
+        cfg.spectrum, cfg.spec_n, cfg.spec_edges, cfg.spec_weights = SPEC_TABLE, weights.size, _p(edges), _p(weights)
     edep = np.zeros(mat.size)
     bs = np.zeros(mat.size) if nbatch > 0 else None
     bs2 = np.zeros(mat.size) if nbatch > 0 else None
@@ -195,6 +203,16 @@ def distance(geom_mode, lbox, r, v, n=(0, 0, 0)):
     bb = (ctypes.c_int * 3)()
     d = lib().oracle_distance(int(geom_mode), float(lbox), int(n[0]), int(n[1]), int(n[2]), rr, vv, bb)
     return d, tuple(bb), tuple(rr)
+
+
+def table_energy(edges, weights, u):
+    """The oracle's table-spectrum energies for uniforms u."""
+    edges = np.ascontiguousarray(edges, dtype=np.float64)
+    weights = np.ascontiguousarray(weights, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    e = np.zeros(u.size)
+    lib().oracle_table_energy(weights.size, _p(edges), _p(weights), u.size, _p(u), _p(e))
+    return e
 
 
 def rotate(uvw, axis):

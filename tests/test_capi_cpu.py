@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert dpm_amd.lib().dpm_abi_version() == 1
+    assert dpm_amd.lib().dpm_abi_version() == 2
 
 
 def test_missing_table_directory_is_an_io_error(tmp_path):

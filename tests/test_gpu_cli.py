@@ -92,3 +92,31 @@ def test_dpm_voxeldose_cli_beamlet_csr(gpu_tables0, tables0, tmp_path):
     r2, c2, v2, _ = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, seed=4, precision=dpm_amd.DPM_FP64)
     assert np.array_equal(row_ptr, r2) and np.array_equal(col, c2)
     np.testing.assert_allclose(val, v2, rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_dpm_voxeldose_cli_spectrum_file(gpu_tables0, tables0, tmp_path):
+    # -S spectrum file ("E_low E_high weight" per bin): the same dose as the
+    # library call with the same table spectrum
+    nx = ny = nz = 12
+    lbox, n = 2.0, 5000
+    spec = tmp_path / "spectrum.txt"
+    spec.write_text("# E_low E_high weight (MeV)\n0.5 1.0 1.0\n1.0 2.0 3.0\n2.0 4.0 0.0\n4.0 6.0 2.5\n6.0 15.0 0.5\n")
+    out = tmp_path / "dose.bin"
+    subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", str(nx), "-y", str(ny), "-z", str(nz), "-l",
+                    str(lbox), "-p", "gamma", "-S", str(spec), "-F", "16", "-n", str(n), "-k", "4", "-s", "5", "-f",
+                    "-o", str(out)], check=True, capture_output=True, timeout=120)
+    d = dpm_amd.read_voxel_dose(str(out))
+    mat = np.zeros((nz, ny, nx), np.uint8)
+    rho = np.ones((nz, ny, nx), np.float32)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, lbox)
+    table = (np.array([0.5, 1.0, 2.0, 4.0, 6.0, 15.0]), np.array([1.0, 3.0, 0.0, 2.5, 0.5]))
+    edep, _, _, _ = dpm_amd.simulate(gpu_tables0, ph, n, particle=0, field=16.0, seed=5, nbatch=4,
+                                     precision=dpm_amd.DPM_FP64, spectrum_table=table)
+    mass_kg = rho.astype(np.float64) * (lbox * 0.1) ** 3 * 1e-3
+    np.testing.assert_allclose(d["dose"], edep * MEV_J / mass_kg / n, rtol=1e-9, atol=1e-30)
+    bad = tmp_path / "gap.txt"
+    bad.write_text("0.5 1.0 1.0\n1.5 2.0 1.0\n")
+    r = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", "4", "-y", "4", "-z", "4", "-S", str(bad),
+                        "-n", "10", "-o", str(tmp_path / "x.bin")], capture_output=True, timeout=60)
+    assert r.returncode != 0

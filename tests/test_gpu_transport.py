@@ -496,3 +496,34 @@ def test_batch_rows_reused_clean_across_runs(gpu_tables0):
         d3, q3, _, _ = dpm_amd.simulate(gpu_tables0, ph, 20000, **y)   # straight after a batch run
         np.testing.assert_allclose(d3, d1, rtol=1e-11, atol=1e-12 * d1.max())
         np.testing.assert_allclose(q3, q1, rtol=1e-11, atol=1e-12 * q1.max())
+
+
+@pytest.mark.gpu
+def test_table_spectrum_source(gpu_tables0, tables0):
+    # a spectrum file (histogram bins of relative weight, one with weight 0):
+    # FP64 GPU transport follows the oracle history by history, and the
+    # primaries' mean energy is the histogram's mean
+    mat, rho = water_cube(30)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    edges = np.array([0.5, 1.0, 2.0, 4.0, 6.0, 15.0])
+    weights = np.array([1.0, 3.0, 0.0, 2.5, 0.5])
+    n = 2000
+    kw = dict(particle=0, field=40.0, seed=17, spectrum_table=(edges, weights))
+    _, _, st, he = dpm_amd.simulate(gpu_tables0, ph, n, precision=dpm_amd.DPM_FP64, per_history=True, **kw)
+    ref = oracle_lib.run_voxel(tables0, mat, rho, 2.0, n, ptype=0, field=40.0, seed=17, per_history=True,
+                               spectrum_table=(edges, weights))
+    rel = np.abs(he - ref["hist_edep"]) / np.maximum(np.abs(ref["hist_edep"]), 1e-300)
+    ok = (rel < 1e-9) | ((np.abs(he) < 1e-300) & (np.abs(ref["hist_edep"]) < 1e-300))
+    assert np.mean(ok) >= 0.99, np.mean(ok)
+    np.testing.assert_allclose(st["e_primary"], ref["e_primary"], rtol=1e-12)
+    # mean primary energy over many histories (FP32) against the histogram's mean
+    m = 200000
+    _, _, st32, _ = dpm_amd.simulate(gpu_tables0, ph, m, **kw)
+    p = weights / weights.sum()
+    mean = np.sum(p * 0.5 * (edges[1:] + edges[:-1]))
+    sd = np.sqrt(np.sum(p * (edges[1:] ** 2 + edges[1:] * edges[:-1] + edges[:-1] ** 2) / 3) - mean ** 2)
+    assert abs(st32["e_primary"] / m - mean) < 5 * sd / np.sqrt(m), (st32["e_primary"] / m, mean)
+    for bad in ((edges, -weights), (edges[::-1].copy(), weights), (edges, 0 * weights),
+                (np.array([0.01, 1.0]), np.array([1.0]))):     # below the photon cut
+        with pytest.raises(dpm_amd.DpmError):
+            dpm_amd.simulate(gpu_tables0, ph, 10, particle=0, field=40.0, spectrum_table=bad)

@@ -95,3 +95,23 @@ def test_rotation_to_lab_frame():
     loc = (0.6, 0.0, 0.8)
     np.testing.assert_allclose(O.rotate(loc, (0, 0, 1)), loc, atol=0)     # identity frame
     np.testing.assert_allclose(O.rotate(loc, (0, 0, -1)), (-0.6, 0.0, -0.8), atol=0)
+
+
+def test_table_spectrum_inverse_cdf():
+    # histogram spectrum, one bin of weight 0: over the whole 2^-24 lattice of
+    # uniforms each bin receives its weight's share of the lattice, energies
+    # never fall in the empty bin, rise with u, and the boundaries of the
+    # cumulative weights map to the edges of the bins they open
+    edges = np.array([0.5, 1.0, 2.0, 4.0, 6.0, 15.0])
+    w = np.array([1.0, 3.0, 0.0, 2.5, 0.5])
+    u = np.arange(0, 1 << 24, 7) / float(1 << 24)
+    e = O.table_energy(edges, w, u)
+    assert np.all(np.diff(e) >= 0) and e[0] == 0.5 and e[-1] < 15.0
+    counts = np.histogram(e, bins=edges)[0] / u.size
+    np.testing.assert_allclose(counts, w / w.sum(), atol=2e-6)
+    cdf = np.concatenate([[0], np.cumsum(w) / w.sum()])
+    # u = cdf[2] = cdf[3] opens bin 3: the empty bin 2 is skipped
+    np.testing.assert_allclose(O.table_energy(edges, w, cdf[[1, 2, 4]]), edges[[1, 3, 4]], rtol=1e-12)
+    # uniform within a bin: the mean of the lattice energies is the histogram mean
+    mean = np.sum(w / w.sum() * 0.5 * (edges[1:] + edges[:-1]))
+    assert abs(e.mean() - mean) < 1e-5
