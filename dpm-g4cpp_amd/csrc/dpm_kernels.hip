@@ -493,8 +493,6 @@ __device__ __forceinline__ R triangular(R u, R lo, R mode, R hi) {
   return hi - rsqrt_(((R)1.0 - u) * (hi - lo) * (hi - mode));
 }
 
-// returns false when a gantry ray misses the phantom (t.E then holds the
-// primary's energy, which the caller counts as escaped)
 // energy from a tabulated spectrum (histogram bins of relative weight):
 // the bin whose cumulative-weight interval [cdf[i], cdf[i+1]) holds the
 // uniform u = k 2^-24, then the same fraction of the bin's width (one uniform,
@@ -512,6 +510,8 @@ __device__ __forceinline__ double spectrum_table(int n, const double* __restrict
   return edges[lo] + f * (edges[lo + 1] - edges[lo]);
 }
 
+// returns false when a gantry ray misses the phantom (t.E then holds the
+// primary's energy, which the caller counts as escaped)
 __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
                                             const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
                                             double& eprim, int k) {
