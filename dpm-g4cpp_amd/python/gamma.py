@@ -3,7 +3,10 @@
 min over evaluated points r' of sqrt((D_e(r') - D_r(r))^2 / (dd * Dmax)^2 +
 |r' - r|^2 / dta^2); a point passes when gamma <= 1. The evaluated
 distribution is searched on a grid refined by linear interpolation so that a
-distance-to-agreement below the voxel size is resolved.
+distance-to-agreement below the voxel size is resolved: offsets of spacing /
+sub within dta (1-D: spacing / 8; 3-D: at most dta / 2). The discrete search
+can only over-estimate gamma (by at most half an offset step over dta), so a
+reported pass rate is a lower bound of the exact one.
 """
 import numpy as np
 from scipy import ndimage
@@ -46,4 +49,8 @@ def gamma_pass_rate_1d(ref, ev, spacing_mm, dd=0.01, dta_mm=1.0, cutoff=0.1):
 
 
 def gamma_pass_rate_3d(ref, ev, spacing_mm, dd=0.01, dta_mm=1.0, cutoff=0.1):
-    return gamma_pass_rate(ref, ev, spacing_mm, dd, dta_mm, cutoff, sub=2)
+    # search offsets at most dta / 2 apart (2 mm voxels: 0.5 mm), so that a
+    # sub-voxel distance to agreement is found; gamma is over-estimated by at
+    # most a quarter of the spatial term's unit
+    sub = max(2, int(np.ceil(2.0 * spacing_mm / dta_mm)))
+    return gamma_pass_rate(ref, ev, spacing_mm, dd, dta_mm, cutoff, sub=sub)

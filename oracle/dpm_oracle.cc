@@ -27,7 +27,12 @@
 namespace {
 
 std::string g_err;
-// test hook: 1 = drop the Compton electron (planted mistake for the tests)
+// test hook: planted mistakes for the tests (0 = none):
+//  1 drop the Compton electron,
+//  2 batch rows shifted by one history (history ih counted in the batch of ih + 1),
+//  3 field source half as wide (x0 + (u - 0.5) * fx / 2),
+//  4 Woodcock interaction probabilities from the unscaled majorant (only the
+//    step length scaled)
 int g_fault = 0;
 
 // ---------------------------------------------------------------------------
@@ -865,8 +870,9 @@ struct Sim {
     const double ecut = g.ecut, gcut = g.gcut;
     while (tr.ekin > 0.0) {
       ++nGStep;
-      const double globalMaxMFP = 1.0 / (g.gmax.value(tr.ekin) * majScale);
+      double globalMaxMFP = 1.0 / (g.gmax.value(tr.ekin) * majScale);
       const double stepLength = -globalMaxMFP * rng.lnu();
+      if (g_fault == 4) globalMaxMFP = 1.0 / g.gmax.value(tr.ekin);
       tr.pos[0] += tr.dir[0] * stepLength;
       tr.pos[1] += tr.dir[1] * stepLength;
       tr.pos[2] += tr.dir[2] * stepLength;
@@ -1094,6 +1100,12 @@ bool gantry_entry(double fu, double fv, double ca, double sa, const double h[3],
   return true;
 }
 
+// the state of a primary as generated (x y z u v w E)
+void store_primary(const Track& p, double* o) {
+  for (int j = 0; j < 3; ++j) { o[j] = p.pos[j]; o[3 + j] = p.dir[j]; }
+  o[6] = p.ekin;
+}
+
 int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
   if (!g.loaded) { g_err = "tables not loaded"; return 1; }
   Sim s;
@@ -1154,7 +1166,8 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
   double ep = 0.0;
   for (uint64_t ih = 0; ih < cfg->nhist; ++ih) {
     if (cfg->nbatch > 0 && geo.mode == ORACLE_GEOM_VOXEL) {
-      const int b = (int)((ih * (uint64_t)cfg->nbatch) / cfg->nhist);
+      const uint64_t ib = g_fault == 2 ? std::min(ih + 1, cfg->nhist - 1) : ih;
+      const int b = (int)((ib * (uint64_t)cfg->nbatch) / cfg->nhist);
       if (b != curBatch) {
         for (int k : geo.touched) { out->batch_sum[k] += batch[k]; out->batch_sum2[k] += batch[k] * batch[k]; batch[k] = 0.0; }
         geo.touched.clear();
@@ -1175,8 +1188,9 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
       s.rng.cur = &p;
       if (cfg->src_kind == ORACLE_SRC_FIELD || cfg->src_kind == ORACLE_SRC_GANTRY) {
         const double ux = s.rng.u(), uy = s.rng.u();
-        p.pos[0] = cfg->x0 + (ux - 0.5) * cfg->fx;
-        p.pos[1] = cfg->y0 + (uy - 0.5) * cfg->fy;
+        const double wf = g_fault == 3 ? 0.5 : 1.0;
+        p.pos[0] = cfg->x0 + (ux - 0.5) * (wf * cfg->fx);
+        p.pos[1] = cfg->y0 + (uy - 0.5) * (wf * cfg->fy);
       }
       if (cfg->spectrum == ORACLE_SPEC_TABLE)
         p.ekin = table_energy(s.rng.u(), spec_cdf, cfg->spec_edges);
@@ -1190,10 +1204,12 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
           // the ray misses the phantom: the primary leaves without interacting
           s.escaped += p.ekin;
           if (out->hist_edep) out->hist_edep[ih] = 0.0;
+          if (out->prim) store_primary(p, out->prim + 7 * ih);
           continue;
         }
       }
     }
+    if (out->prim) store_primary(p, out->prim + 7 * ih);
     while (!s.stack.empty()) {
       Track tr = s.stack.back();
       s.stack.pop_back();

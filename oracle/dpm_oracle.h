@@ -76,6 +76,7 @@ typedef struct oracle_out {
      secondaries created, electron steps, photon steps */
   uint64_t cnt[9];
   double   e_primary;     /* sum of the primaries' kinetic energies */
+  double*  prim;          /* optional [nhist][7]: each primary as generated, x y z u v w E */
 } oracle_out;
 
 int  oracle_load(const char* dir);          /* 0 on success */
@@ -92,7 +93,9 @@ int oracle_sample_many(int kind, int n, double e, int imat, const double* r1, co
                        double* out);
 /* table interpolations as used by the transport (with the reference clamps) */
 double oracle_eval(int which, double e, int imat);
-/* test hook: plant a known mistake (1 = drop the Compton electron), 0 = off */
+/* test hook: plant a known mistake, 0 = off: 1 drop the Compton electron,
+   2 batch rows shifted by one history, 3 field source half as wide,
+   4 Woodcock interaction probabilities from the unscaled majorant */
 void   oracle_set_fault(int f);
 /* Philox4x32-10 block for tests */
 void   oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

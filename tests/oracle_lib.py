@@ -43,7 +43,8 @@ class Out(ctypes.Structure):
                 ("hist_cap", ctypes.c_int), ("hist_size", ctypes.c_int), ("max_iz", ctypes.c_int),
                 ("e_escaped", ctypes.c_double), ("e_scored", ctypes.c_double),
                 ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64),
-                ("cnt", ctypes.c_uint64 * NCNT), ("e_primary", ctypes.c_double)]
+                ("cnt", ctypes.c_uint64 * NCNT), ("e_primary", ctypes.c_double),
+                ("prim", ctypes.c_void_p)]
 
 
 _lib = None
@@ -118,11 +119,13 @@ def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_
 
 def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
               seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None,
-              spectrum_table=None):
+              spectrum_table=None, field_y=None, primaries=False):
     """Voxel run in Philox mode. angle (degrees) selects the gantry source (a
     field, or a pencil when field == 0, whose ray enters the phantom from the
     side the beam direction (sin a, 0, cos a) points away from); emode selects
-    the triangular spectrum (emin, emode, emax)."""
+    the triangular spectrum (emin, emode, emax); field_y (default: field) is
+    the field's y size; primaries=True returns each primary as generated
+    (x y z u v w E per history) under "prim"."""
     load(datadir)
     mat = np.ascontiguousarray(mat, dtype=np.uint8)
     rho = np.ascontiguousarray(rho, dtype=np.float32)
@@ -132,7 +135,8 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
               src_kind=SRC_GANTRY if angle is not None else (SRC_FIELD if field > 0 else SRC_PENCIL),
               angle_deg=0.0 if angle is None else angle, spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT,
               emode=0.0 if emode is None else emode, ptype=ptype, e0=e0, emin=emin, emax=emax,
-              fx=field, fy=field, x0=x0, y0=y0, hist_begin=hist_begin, nhist=nhist, nbatch=nbatch)
+              fx=field, fy=field if field_y is None else field_y, x0=x0, y0=y0, hist_begin=hist_begin,
+              nhist=nhist, nbatch=nbatch)
     if spectrum_table is not None:
         edges = np.ascontiguousarray(spectrum_table[0], dtype=np.float64)
         weights = np.ascontiguousarray(spectrum_table[1], dtype=np.float64)
@@ -141,10 +145,11 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
     bs = np.zeros(mat.size) if nbatch > 0 else None
     bs2 = np.zeros(mat.size) if nbatch > 0 else None
     hed = np.zeros(nhist) if per_history else None
-    out = Out(edep=_p(edep), batch_sum=_p(bs), batch_sum2=_p(bs2), hist_edep=_p(hed))
+    prim = np.zeros((nhist, 7)) if primaries else None
+    out = Out(edep=_p(edep), batch_sum=_p(bs), batch_sum2=_p(bs2), hist_edep=_p(hed), prim=_p(prim))
     if lib().oracle_run(ctypes.byref(cfg), ctypes.byref(out)) != 0:
         raise RuntimeError(lib().oracle_error().decode())
-    return dict(edep=edep.reshape(mat.shape), batch_sum=bs, batch_sum2=bs2, hist_edep=hed,
+    return dict(edep=edep.reshape(mat.shape), batch_sum=bs, batch_sum2=bs2, hist_edep=hed, prim=prim,
                 e_escaped=out.e_escaped, e_scored=out.e_scored, n_pair=out.n_pair, n_annih=out.n_annih,
                 counters=dict(zip(COUNTER_NAMES, list(out.cnt))), e_primary=out.e_primary)
 
@@ -183,9 +188,27 @@ def evaluate(which, e, imat=0):
     return lib().oracle_eval(which, e, imat)
 
 
+FAULT_NONE, FAULT_COMPTON_ELECTRON, FAULT_BATCH_ROW, FAULT_FIELD_WIDTH, FAULT_MAJORANT = 0, 1, 2, 3, 4
+
+
 def set_fault(f):
-    """Plant a known mistake in the oracle (1 = drop the Compton electron)."""
+    """Plant a known mistake in the oracle: 1 drop the Compton electron, 2 batch
+    rows shifted by one history, 3 field source half as wide, 4 Woodcock
+    interaction probabilities from the unscaled majorant; 0 = none."""
     lib().oracle_set_fault(int(f))
+
+
+class fault:
+    """Context manager planting oracle mistake `f` for the duration of a block."""
+
+    def __init__(self, f):
+        self.f = f
+
+    def __enter__(self):
+        set_fault(self.f)
+
+    def __exit__(self, *exc):
+        set_fault(0)
 
 
 def philox(ctr, key):
