@@ -589,6 +589,7 @@ struct dpm_phantom {
   uint8_t* d_mat = nullptr;
   float* d_rho = nullptr;
   double maj_scale = 1.0;
+  double maj_min = 1.0;     // smallest valid scale: max(rho_voxel / rho_nominal), >= 1
   int nbins = 0;
   int64_t nvox = 0;
 };
@@ -609,6 +610,7 @@ void fill_geometry(const dpm_tables* t, const dpm_phantom* p, Geometry<R>& G) {
   G.vmat = p->d_mat;
   G.vrho = p->d_rho;
   for (int im = 0; im < t->nmat; ++im) G.rho[im] = (R)t->rho[im];
+  G.guard = &t->stats->guard_push;
 }
 
 // device-time of a pair of events, 0 when unavailable
@@ -1094,6 +1096,7 @@ int finish_stats(dpm_tables* t, dpm_stats* st) {
     st->e_primary = h.e_primary;
     st->e_scored = h.scored;
     st->hist_size = std::max(20, h.maxiz + 10);
+    st->guard_exits = h.cnt[0][kCntGuard] + h.cnt[1][kCntGuard] + h.guard_push;
   }
   return DPM_OK;
 }
@@ -1346,6 +1349,7 @@ int dpm_phantom_create_voxel(const dpm_tables* t, int nx, int ny, int nz, double
   p->nx = nx; p->ny = ny; p->nz = nz;
   p->nvox = nv;
   p->maj_scale = scale;
+  p->maj_min = scale;
   if (hipMalloc(&p->d_mat, nv) != hipSuccess || hipMalloc(&p->d_rho, nv * sizeof(float)) != hipSuccess ||
       hipMemcpy(p->d_mat, mat, nv, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(p->d_rho, rho, nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
@@ -1355,6 +1359,16 @@ int dpm_phantom_create_voxel(const dpm_tables* t, int nx, int ny, int nz, double
   *out = p;
   return DPM_OK;
 }
+
+int dpm_phantom_set_majorant(dpm_phantom* p, double scale) {
+  if (!p) return set_error(DPM_E_INVALID, "null argument");
+  if (!std::isfinite(scale) || scale < p->maj_min)
+    return set_error(DPM_E_INVALID, "majorant scale below max(rho / rho_nominal) = " + std::to_string(p->maj_min));
+  p->maj_scale = scale;
+  return DPM_OK;
+}
+
+double dpm_phantom_get_majorant(const dpm_phantom* p) { return p ? p->maj_scale : 0.0; }
 
 void dpm_phantom_destroy(dpm_phantom* p) {
   if (!p) return;
@@ -1758,7 +1772,7 @@ int dpm_beamlets_fetch(dpm_tables* t, int32_t* col, double* val) {
 int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* energy, const int* imat,
                const uint32_t* r1, const uint32_t* r2, const uint32_t* r3, double* out, int32_t* ebin, int32_t* abin) {
   if (!t || !energy || !r1 || !r2 || !out || !ebin || !abin || n < 0) return set_error(DPM_E_INVALID, "null argument");
-  if (kind < 0 || kind > 3) return set_error(DPM_E_INVALID, "unknown sampler");
+  if (kind < 0 || kind > 4) return set_error(DPM_E_INVALID, "unknown sampler");
   if (kind <= 2 && !r3) return set_error(DPM_E_INVALID, "sampler needs three uniforms");
   if (kind == 0 && !imat) return set_error(DPM_E_INVALID, "SB sampler needs material indices");
   if (n == 0) return DPM_OK;

@@ -428,6 +428,7 @@ struct Geometry {
   const uint8_t* vmat;            // [nz][ny][nx]
   const float* vrho;
   R rho[kMaxMat];                 // nominal densities (slab)
+  unsigned long long* guard;      // counts tolerance-push loops that end without a decision (or null)
 };
 
 // one pass of the boundary computation; returns the push axis (0..2) and
@@ -487,7 +488,9 @@ __device__ __forceinline__ R distance_to_boundary(const Geometry<R>& g, R r[3], 
     else if (push == 1) r[1] += d;
     else r[2] += d;
   }
-  return (R)-1.0;   // pathological: treat as leaving the geometry
+  // pathological: treat as leaving the geometry, and count it
+  if (g.guard) atomicAdd(g.guard, 1ull);
+  return (R)-1.0;
 }
 
 // DistanceToBoundary where only the box, the tolerance push and the extent
@@ -532,7 +535,9 @@ __device__ __forceinline__ bool locate(const Geometry<R>& g, R r[3], const R v[3
     else if (push == 1) r[1] += d;
     else r[2] += d;
   }
-  return false;   // pathological: treat as leaving the geometry
+  // pathological: treat as leaving the geometry, and count it
+  if (g.guard) atomicAdd(g.guard, 1ull);
+  return false;
 }
 
 // box of a position without the extent check (the reference keeps the

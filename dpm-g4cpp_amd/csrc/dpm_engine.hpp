@@ -10,10 +10,11 @@ namespace dpm {
 
 // counters accumulated by the kernels (the first ten are exported through
 // dpm_stats.counters; deposits counts scoring calls, tally atomics the FP64
-// atomics actually issued after write-combining)
+// atomics actually issued after write-combining; guard counts particles ended
+// by the per-particle iteration cap)
 enum {
   kCntVoxelVisits = 0, kCntDeposits, kCntSB, kCntMoller, kCntKN, kCntGS, kCntSecondaries, kCntESteps, kCntGSteps,
-  kCntTallyAtomics, kCntPair, kCntAnnih, kCntLaneSlots, kNumCounters
+  kCntTallyAtomics, kCntPair, kCntAnnih, kCntLaneSlots, kCntGuard, kNumCounters
 };
 
 // counters per kernel kind: 0 photons, 1 electrons
@@ -24,6 +25,7 @@ struct StatsDev {
   double scored;            // sum of the returned tally (set after the run)
   int maxiz;
   int pad;
+  unsigned long long guard_push;   // boundary / locate loops that ran out of tolerance pushes
 };
 
 // Particle queue in blocks of 64 records (AoSoA): block b holds slots

@@ -666,7 +666,9 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, co
     } else {
       const R e0 = t.E * kInvEMC2;
       const R elCost = ((R)1.0 + e0) * rsqrt_(((R)1.0 - theEps) / (e0 * ((R)2.0 + e0 * ((R)1.0 - theEps))));
-      const R sint = rsqrt_(((R)1.0 + elCost) * ((R)1.0 - elCost));
+      // FP32 rounding can put elCost a few ulp above 1 at eps = eps_min:
+      // the clamp keeps the sine real and leaves FP64 values unchanged
+      const R sint = rsqrt_(rmax((R)0.0, ((R)1.0 + elCost) * ((R)1.0 - elCost)));
       const R phi = phPhi + (R)kPI;
       R sp, cp;
       rsincos(phi, &sp, &cp);
@@ -680,7 +682,7 @@ __device__ __forceinline__ bool photon_step(BlockAcc& a, const KParams<R>& P, co
       score<GEOM>(a, P, t.E, t.box, t.tag, t.rng.hist);
       return false;
     }
-    const R phSint = rsqrt_(((R)1.0 - phCost) * ((R)1.0 + phCost));
+    const R phSint = rsqrt_(rmax((R)0.0, ((R)1.0 - phCost) * ((R)1.0 + phCost)));
     R sp, cp;
     rsincos(phPhi, &sp, &cp);
     R u1 = phSint * cp, u2 = phSint * sp, u3 = phCost;
@@ -822,7 +824,11 @@ k_photons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, QueueV
     if (has) {
       t.rng.topup();
       has = photon_step<GEOM, LDS>(acc, P, PT, t, oe, og);
-      if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
+      if (has && ++t.iter > kMaxIter) {
+        count(acc, kCntGuard);
+        escape(acc, t.E);
+        has = false;
+      }
     }
   }
   if (lane_id() == 0) atomicAdd(&acc.cnt[kCntLaneSlots], 64u * trips);
@@ -1199,7 +1205,11 @@ k_electrons(KParams<R> P, const SourceParams* __restrict__ S, PrimRange pr, Queu
     if (has) {
       t.rng.topup();
       has = electron_step<GEOM, LDS>(acc, P, SP, t, s, wc, hot, wave, oe, og);
-      if (has && ++t.iter > kMaxIter) { escape(acc, t.E); has = false; }
+      if (has && ++t.iter > kMaxIter) {
+        count(acc, kCntGuard);
+        escape(acc, t.E);
+        has = false;
+      }
       if (!has) flush_wc(acc, P, wc, hot, wave);
     }
   }
@@ -1225,6 +1235,7 @@ __global__ void k_sample(Tables<R> T, int kind, int n, const double* e, const in
   else if (kind == 1) v = (double)sample_moller(T.mo, E, r1[i], r2[i], r3[i], &eb, &ab);
   else if (kind == 2) v = (double)sample_kn(T.kn, E, r1[i], r2[i], r3[i], &eb, &ab);
   else if (kind == 3) v = 1.0 - (double)sample_gs_omu(T.gs, E, r1[i], r2[i], &eb, &ab);
+  else if (kind == 4) v = (double)sample_gs_omu(T.gs, E, r1[i], r2[i], &eb, &ab);
   out[i] = v;
   ebin[i] = eb;
   abin[i] = ab;
@@ -1274,7 +1285,7 @@ __global__ void k_perform(Tables<R> T, int kind, int n, double energy, int imat,
     const R eps = sample_kn(T.kn, E, q1, q2, q3, (int*)nullptr, (int*)nullptr);
     cost = (R)1.0 - ((R)1.0 - eps) / (eps * E * (R)(1.0 / kEMC2));
     v = eps * E;
-    sint = rsqrt_(((R)1.0 - cost) * ((R)1.0 + cost));
+    sint = rsqrt_(rmax((R)0.0, ((R)1.0 - cost) * ((R)1.0 + cost)));
   }
   // azimuth and rotation to the frame of a primary along +z (the identity)
   const R phi = (R)(2.0 * kPI) * rng.uni<R>();

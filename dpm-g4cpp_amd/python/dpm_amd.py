@@ -26,7 +26,7 @@ EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_t
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
             "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_simulate_fields", "dpm_last_error",
-            "dpm_abi_version", "dpm_probe_atomics")
+            "dpm_abi_version", "dpm_probe_atomics", "dpm_phantom_set_majorant", "dpm_phantom_get_majorant")
 
 
 class DpmError(RuntimeError):
@@ -60,7 +60,7 @@ class Stats(ctypes.Structure):
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
                 ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int), ("lds_tables", ctypes.c_int),
-                ("lane_slots", ctypes.c_uint64 * 2)]
+                ("lane_slots", ctypes.c_uint64 * 2), ("guard_exits", ctypes.c_uint64)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
@@ -70,7 +70,7 @@ class Stats(ctypes.Structure):
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
                     kernel_calls=list(self.kernel_calls), graphs=self.graphs, lds_tables=self.lds_tables,
-                    lane_slots=list(self.lane_slots))
+                    lane_slots=list(self.lane_slots), guard_exits=self.guard_exits)
 
 
 class Schedule(ctypes.Structure):
@@ -124,6 +124,9 @@ def lib():
                                             ctypes.POINTER(Schedule)]
         L.dpm_host_free.argtypes = [vp]
         L.dpm_probe_atomics.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp]
+        L.dpm_phantom_set_majorant.argtypes = [vp, dp]
+        L.dpm_phantom_get_majorant.argtypes = [vp]
+        L.dpm_phantom_get_majorant.restype = dp
         _lib = L
     return _lib
 
@@ -157,8 +160,9 @@ class Tables:
             pass
 
     def sample(self, kind, energy, r1, r2, r3=None, imat=None, precision=DPM_FP32):
-        """Sampler with explicit 24-bit integer uniforms; kind in sb|mo|kn|gs."""
-        k = {"sb": 0, "mo": 1, "kn": 2, "gs": 3}[kind]
+        """Sampler with explicit 24-bit integer uniforms; kind in sb|mo|kn|gs
+        (cos theta)|gs_omu (1 - cos theta)."""
+        k = {"sb": 0, "mo": 1, "kn": 2, "gs": 3, "gs_omu": 4}[kind]
         e = np.ascontiguousarray(energy, dtype=np.float64)
         n = e.size
         u1 = np.ascontiguousarray(r1, dtype=np.uint32)
@@ -212,6 +216,15 @@ class Phantom:
         _check(lib().dpm_phantom_create_voxel(tables.h, nx, ny, nz, lbox, _ptr(mat), _ptr(rho), ctypes.byref(h)))
         return cls(tables, h, mat.shape)
 
+    @property
+    def majorant(self):
+        """Woodcock majorant scale of the photon steps (dpm_phantom_get_majorant)."""
+        return lib().dpm_phantom_get_majorant(self.h)
+
+    @majorant.setter
+    def majorant(self, scale):
+        _check(lib().dpm_phantom_set_majorant(self.h, float(scale)))
+
     def close(self):
         if self.h:
             lib().dpm_phantom_destroy(self.h)
@@ -260,7 +273,7 @@ def pinned_empty(n, dtype=np.float64):
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
              out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
-             angle=None, emode=None, table_memory=0, pipes=0, spectrum_table=None):
+             angle=None, emode=None, table_memory=0, pipes=0, spectrum_table=None, field_y=None):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -268,8 +281,10 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     reshaped to the phantom's shape. fetch=False leaves the result on the
     device (tally and steps are returned as None; see tally_fetch). angle
     (degrees) selects the gantry source (voxel phantoms), emode the triangular
-    spectrum (emin, emode, emax)."""
+    spectrum (emin, emode, emax); field_y (default: field) the field's y size."""
     src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode, spectrum_table)
+    if field_y is not None:
+        src.field_y = field_y
     he = np.zeros(nhist) if per_history else None
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],

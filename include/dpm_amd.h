@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DPM_ABI_VERSION 2
+#define DPM_ABI_VERSION 3
 
 enum {
   DPM_OK = 0,
@@ -125,6 +125,16 @@ int  dpm_phantom_create_slab(const dpm_tables* t, double lbox, int geom_index, d
 int  dpm_phantom_create_voxel(const dpm_tables* t, int nx, int ny, int nz, double lbox,
                               const uint8_t* mat, const float* rho, dpm_phantom** out);
 void dpm_phantom_destroy(dpm_phantom* p);
+/* Woodcock majorant scale of the photon steps (KeepTrackingPhoton,
+ * SimDPMLike.cc:467-510, tracks against the table maximum
+ * SimIMFPMaxPhoton::GetIMFP scaled by this factor). A voxel phantom sets it
+ * to max(rho_voxel / rho_nominal) >= 1, so that per-voxel densities above
+ * the nominal one stay under the majorant; the slab keeps 1. Any larger
+ * scale gives the same dose distribution with more null events; a smaller
+ * one is rejected. No counterpart in the reference (whose densities are the
+ * nominal ones). */
+int dpm_phantom_set_majorant(dpm_phantom* p, double scale);
+double dpm_phantom_get_majorant(const dpm_phantom* p);
 /* number of tally bins: depth bins (slab) or voxels */
 int64_t dpm_phantom_num_bins(const dpm_phantom* p);
 
@@ -217,6 +227,10 @@ typedef struct dpm_stats {
   uint64_t lane_slots[2];          /* photon / electron kernels: 64 per transport-loop trip of a
                                       wave; the steps (counters_*[DPM_CNT_G_STEPS / _E_STEPS]) over
                                       this are the lane utilisation */
+  uint64_t guard_exits;            /* particles ended by a termination guard instead of the physics:
+                                      the per-particle cap of 2e5 transport steps, or a boundary /
+                                      locate loop out of tolerance pushes (16); their energy is counted
+                                      as escaped. 0 in every run that follows the reference */
 } dpm_stats;
 
 /* Replaces Simulate(nprimary, eprimary, iselectron, lbox, mat, el, ph,
@@ -311,6 +325,8 @@ int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
  *   kind 1 Moller: SimMollerTables::SampleEnergyTransfer(E, r1, r2, r3)    SimMollerTables.cc:69
  *   kind 2 KN    : SimKNTables::SampleEnergyTransfer(E, r1, r2, r3)        SimKNTables.cc:69
  *   kind 3 GS    : SimGSTables::SampleAngularDeflection(E, r1, r2)         SimGSTables.cc:84
+ *   kind 4 GS    : the same sample as 1 - cos(theta) (the quantity the MSC
+ *                  hinge applies; free of the cancellation of 1 - out[i] of kind 3)
  * out[i] is the sampled value; ebin[i] the chosen energy-grid table and
  * abin[i] the alias (or cumulative) bin, for exactness checks. */
 int dpm_sample(dpm_tables* t, int kind, int precision, int n, const double* energy, const int* imat,

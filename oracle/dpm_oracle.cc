@@ -432,7 +432,9 @@ double sample_kn(double egamma, double r1, double r2, double r3, int* ebin, int*
 }
 
 // SimGSTables::SampleAngularDeflection (SimGSTables.cc:84-119)
-double sample_gs(double eprim, double r1, double r2, int* ebin, int* cbin) {
+// 1 - cos(theta) of SimGSTables::SampleAngularDeflection, i.e. the
+// reference's 2 a U / (1 - U + a) before it is subtracted from 1
+double sample_gs_omu(double eprim, double r1, double r2, int* ebin, int* cbin) {
   const GSTable& t = g.gs;
   const double lpe = std::log(eprim);
   double ph = (lpe - t.logMinE) * t.invLogDelta;
@@ -452,7 +454,12 @@ double sample_gs(double eprim, double r1, double r2, int* ebin, int* cbin) {
   const double a0 = t.transf[ielow], a1 = t.transf[ielow + 1];
   const double e0 = t.egrid[ielow], e1 = t.egrid[ielow + 1];
   const double parTransf = (a1 - a0) / (e1 - e0) * (eprim - e0) + a0;
-  return 1. - 2. * parTransf * theU / (1. - theU + parTransf);
+  return 2. * parTransf * theU / (1. - theU + parTransf);
+}
+
+// SimGSTables::SampleAngularDeflection (SimGSTables.cc:84-119): cos(theta)
+double sample_gs(double eprim, double r1, double r2, int* ebin, int* cbin) {
+  return 1. - sample_gs_omu(eprim, r1, r2, ebin, cbin);
 }
 
 // ---------------------------------------------------------------------------
@@ -1249,6 +1256,9 @@ double oracle_sample_kn(double e, double r1, double r2, double r3, int* ebin, in
 }
 double oracle_sample_gs(double e, double r1, double r2, int* ebin, int* cbin) {
   return sample_gs(e, r1, r2, ebin, cbin);
+}
+double oracle_sample_gs_omu(double e, double r1, double r2, int* ebin, int* cbin) {
+  return sample_gs_omu(e, r1, r2, ebin, cbin);
 }
 
 // n samples at one energy with explicit FP64 uniforms (kind 0 SB, 1 Moller,

@@ -89,6 +89,8 @@ double oracle_sample_sb(double e, int imat, double r1, double r2, double r3, int
 double oracle_sample_moller(double e, double r1, double r2, double r3, int* ebin, int* abin);
 double oracle_sample_kn(double e, double r1, double r2, double r3, int* ebin, int* abin);
 double oracle_sample_gs(double e, double r1, double r2, int* ebin, int* cbin);
+/* the same GS sample as 1 - cos(theta) (no cancellation for small angles) */
+double oracle_sample_gs_omu(double e, double r1, double r2, int* ebin, int* cbin);
 int oracle_sample_many(int kind, int n, double e, int imat, const double* r1, const double* r2, const double* r3,
                        double* out);
 /* table interpolations as used by the transport (with the reference clamps) */

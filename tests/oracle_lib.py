@@ -69,6 +69,8 @@ def lib():
             f.argtypes = args + [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_sample_gs.restype = ctypes.c_double
         L.oracle_sample_gs.argtypes = [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2
+        L.oracle_sample_gs_omu.restype = ctypes.c_double
+        L.oracle_sample_gs_omu.argtypes = [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2
         dptr = ctypes.POINTER(ctypes.c_double)
         L.oracle_sample_many.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dptr, dptr, dptr,
                                          dptr]
@@ -168,7 +170,8 @@ def sample_many(kind, e, r, imat=0):
 
 
 def sample(kind, e, r, imat=0):
-    """Reference-order sampler call with explicit uniforms; returns (value, ebin, abin)."""
+    """Reference-order sampler call with explicit uniforms; returns (value, ebin, abin).
+    kind in sb|mo|kn|gs (cos theta)|gs_omu (1 - cos theta)."""
     eb, ab = ctypes.c_int(-1), ctypes.c_int(-1)
     L = lib()
     if kind == "sb":
@@ -179,6 +182,8 @@ def sample(kind, e, r, imat=0):
         v = L.oracle_sample_kn(e, r[0], r[1], r[2], ctypes.byref(eb), ctypes.byref(ab))
     elif kind == "gs":
         v = L.oracle_sample_gs(e, r[0], r[1], ctypes.byref(eb), ctypes.byref(ab))
+    elif kind == "gs_omu":
+        v = L.oracle_sample_gs_omu(e, r[0], r[1], ctypes.byref(eb), ctypes.byref(ab))
     else:
         raise ValueError(kind)
     return v, eb.value, ab.value

@@ -31,7 +31,29 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert dpm_amd.lib().dpm_abi_version() == 2
+    assert dpm_amd.lib().dpm_abi_version() == 3
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizes and member offsets of the ctypes mirrors against the C compiler's
+    # view of include/dpm_amd.h
+    structs = {"dpm_stats": dpm_amd.Stats, "dpm_run_options": dpm_amd.RunOptions, "dpm_source": dpm_amd.Source,
+               "dpm_schedule": dpm_amd.Schedule, "dpm_beamlet": dpm_amd.Beamlet, "dpm_field": dpm_amd.Field}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dpm_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    c = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, py in structs.items():
+        assert int(c[cname]) == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert int(c["%s.%s" % (cname, f[0])]) == getattr(py, f[0]).offset, (cname, f[0])
 
 
 def test_missing_table_directory_is_an_io_error(tmp_path):

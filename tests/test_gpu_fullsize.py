@@ -90,3 +90,55 @@ def test_config5_full_size(gpu_tables3):
     print("config 5 full size: gamma FP32 vs FP64 (central 128^3) %.5f" % pr, flush=True)
     assert pr >= 0.99, pr
     ph.close()
+
+
+@pytest.mark.gpu
+def test_config4_full_size_beamlets(gpu_tables0):
+    # 512 beamlets (e- 6-18 MeV, photons U(1, 20) MeV, entry points on the
+    # 2 mm grid over 160x160 mm2, lognormal history counts) on the 150^3 water
+    # phantom in one batched FP32 run; properties of the CSR dose-influence
+    # matrix that hold at any size, and four rows against plain runs
+    mat, rho = phantoms.water_cube(150)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    bl = phantoms.beamlets(512, seed=4321)
+    thr = 1e-3
+    total = sum(b[4] for b in bl)
+    print("config 4 full size: %d beamlets, %d histories" % (len(bl), total), flush=True)
+    row_ptr, col, val, st = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, rel_threshold=thr, seed=8)
+    assert st["guard_exits"] == 0
+    # energy balance of the whole run
+    bal = _balance(st, st["e_primary"])
+    assert abs(bal) < 1e-5 * st["e_primary"], bal
+    np.testing.assert_allclose(st["e_primary"], sum(b[3] * b[4] for b in bl), rtol=1e-6)
+    # CSR structure: rows in order, voxel indices valid and unique per row,
+    # every kept value above the threshold of its row's maximum
+    assert row_ptr[0] == 0 and np.all(np.diff(row_ptr) >= 1) and row_ptr[-1] == col.size
+    assert col.min() >= 0 and col.max() < ph.nbins and np.all(val > 0)
+    csr_total = 0.0
+    for b in range(len(bl)):
+        c, v = col[row_ptr[b]:row_ptr[b + 1]], val[row_ptr[b]:row_ptr[b + 1]]
+        assert np.unique(c).size == c.size
+        assert v.min() >= thr * v.max() * (1 - 1e-12)
+        csr_total += v.sum()
+    # the thresholded rows hold part of the scored energy, never more
+    assert csr_total <= st["e_scored"] * (1 + 1e-9)
+    assert csr_total > 0.5 * st["e_scored"]
+    # four rows (the largest, the smallest, an electron and a photon beamlet)
+    # against plain per-beamlet runs of the same history ids
+    off = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
+    counts = np.array([b[4] for b in bl])
+    picks = {int(np.argmax(counts)), int(np.argmin(counts)),
+             next(i for i, b in enumerate(bl) if b[2] == -1), next(i for i, b in enumerate(bl) if b[2] == 0)}
+    for b in sorted(picks):
+        x, y, p, e, n = bl[b]
+        d, _, _, _ = dpm_amd.simulate(gpu_tables0, ph, n, particle=p, energy=e, x0=x, y0=y, seed=8,
+                                      hist_begin=int(off[b]))
+        d = d.ravel()
+        keep = np.nonzero((d > thr * d.max()) & (d > 0))[0]
+        c, v = col[row_ptr[b]:row_ptr[b + 1]], val[row_ptr[b]:row_ptr[b + 1]]
+        o = np.argsort(c)
+        np.testing.assert_array_equal(c[o], keep)
+        np.testing.assert_allclose(v[o], d[keep], rtol=1e-9)
+        print("config 4: beamlet %d (%s %.1f MeV, %d histories) equals its plain run: %d voxels"
+              % (b, "e-" if p == -1 else "gamma", e, n, keep.size), flush=True)
+    ph.close()

@@ -113,3 +113,32 @@ def test_eval_matches_reference(gpu_tables3, gold):
 def test_sampler_rejects_bad_uniforms(gpu_tables3):
     with pytest.raises(dpm_amd.DpmError):
         gpu_tables3.sample("kn", np.array([1.0]), np.array([1 << 24]), np.array([0]), np.array([0]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,bound", [(dpm_amd.DPM_FP32, 1e-6), (dpm_amd.DPM_FP64, 1e-12)])
+def test_gs_small_deflections_relative(gpu_tables3, tables3, gold, precision, bound):
+    """1 - cos(theta) of the GS sampler, the quantity the MSC hinge applies,
+    within a relative bound of the FP64 oracle's 2 a U / (1 - U + a) at the
+    same (float) energy -- including the smallest deflections (uniforms in the
+    first cumulative bins, high energies), where an absolute bound on cos(theta)
+    says nothing."""
+    rng = np.random.default_rng(77)
+    e_fix = gold["gs_e"]
+    n = 20000
+    e = np.exp(rng.uniform(np.log(max(e_fix.min(), 1.0)), np.log(e_fix.max()), n)).astype(np.float32).astype(np.float64)
+    k1 = rng.integers(0, 1 << 24, n, dtype=np.uint32)
+    # half of the draws in the first 1/64 of the cumulative range (smallest angles)
+    k2 = np.where(rng.uniform(size=n) < 0.5, rng.integers(0, 1 << 18, n), rng.integers(0, 1 << 24, n)).astype(np.uint32)
+    out, eb, ab = gpu_tables3.sample("gs_omu", e, k1, k2, None, precision=precision)
+    oracle_lib.load(tables3)
+    ref = np.empty(n)
+    for i in range(n):
+        v, reb, rab = oracle_lib.sample("gs_omu", float(e[i]), (k1[i] / float(1 << 24), k2[i] / float(1 << 24)))
+        assert eb[i] == reb and ab[i] == rab, (i, e[i], eb[i], reb, ab[i], rab)
+        ref[i] = v
+    assert np.all(ref > 0.0)
+    rel = np.abs(out - ref) / ref
+    print("GS 1-cos(theta): smallest %.3e, largest relative error %.3e" % (ref.min(), rel.max()))
+    assert ref.min() < 1e-5          # the small-angle region is reached
+    assert rel.max() <= bound, (rel.max(), int(np.argmax(rel)))
