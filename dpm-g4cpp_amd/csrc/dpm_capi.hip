@@ -19,6 +19,16 @@
 
 using namespace dpm;
 
+// Device scratch of dpm_simulate_noalloc: the source description of a run
+// (the first kScratchSrcBytes) and its four particle queues (the rest). One
+// copy per GPU, held from the library's load on that GPU until the process
+// ends. In host code `scratch` only names the symbol: the device address of a
+// GPU's copy comes from hipGetSymbolAddress with that GPU current
+// (dpm_tables_create keeps it in the tables handle).
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+constexpr size_t kScratchSrcBytes = (size_t)64 << 10;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -265,19 +275,40 @@ struct Workspace {
   void* buf = nullptr;
   size_t bytes = 0;
   int cap = 0;
+  bool fixed = false;  // a view of memory owned elsewhere (see view()): never reallocated
   QueueView<R> q[4];   // G0, G1, E0, E1
 
   void release() {
-    if (buf) (void)hipFree(buf);
+    if (buf && !fixed) (void)hipFree(buf);
     buf = nullptr;
     bytes = 0;
     cap = 0;
+  }
+  // records each of the 4 queues holds when they share `total` bytes
+  static int capacity_of(size_t total) {
+    const size_t per = (total / 4) & ~(size_t)255;
+    return (int)std::min<size_t>(per / kQBlockBytes<R> * kQLanes, (size_t)1 << 30);
+  }
+  // the 4 queues laid out in [base, base + total) without allocating
+  void view(unsigned char* base, size_t total) {
+    buf = base;
+    bytes = total;
+    fixed = true;
+    cap = capacity_of(total);
+    const size_t per = (total / 4) & ~(size_t)255;
+    for (int k = 0; k < 4; ++k) {
+      q[k].base = base + k * per;
+      q[k].cap = cap;
+    }
   }
   int ensure(int want, int* ctl) {
     if (cap >= want && buf) {
       for (int k = 0; k < 4; ++k) q[k].count = (unsigned long long*)ctl + k;
       return DPM_OK;
     }
+    if (fixed)
+      return set_error(DPM_E_NOMEM, "particle queues need " + std::to_string(want) + " records, the fixed workspace holds " +
+                                        std::to_string(cap));
     release();
     want = (want + kQLanes - 1) / kQLanes * kQLanes;
     const size_t per = (size_t)(want / kQLanes) * kQBlockBytes<R>;
@@ -440,6 +471,13 @@ __global__ void k_sum(const double* x, int64_t n, double* out) {
   }
 }
 
+// clears n doubles (a kernel, not a memset, so that a caller may record the
+// no-allocation run into a graph)
+__global__ void k_zero(double* x, int64_t n) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    x[v] = 0.0;
+}
+
 // Beamlet rows -> CSR in three passes over (row, segment) blocks, 1-D grid
 // block = row * kCsrSeg + segment:
 //   k_bl_stats  row maxima (of the non-negative part) and the sum of the rows
@@ -543,6 +581,7 @@ __global__ void k_bl_fill(const double* rows, int64_t nvox, const unsigned long 
 // ---------------------------------------------------------------------------
 struct dpm_tables {
   int device = 0;
+  unsigned char* scratch_dev = nullptr;   // this device's copy of `scratch` (dpm_simulate_noalloc)
   hipStream_t stream = nullptr;
   int num_cu = 0;
   int nmat = 0;
@@ -692,8 +731,15 @@ void gantry_cos_sin(double deg, double* c, double* s) {
   *s = std::sin(a);
 }
 
+// Fixed device memory a run uses instead of allocating (the no-allocation
+// entry point): the source description goes to [src, src + src_bytes).
+struct FixedMem {
+  unsigned char* src = nullptr;
+  size_t src_bytes = 0;
+};
+
 int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, uint64_t hist_begin, uint64_t nhist,
-                  SourceParams** out) {
+                  SourceParams** out, const FixedMem* fm = nullptr) {
   size_t bytes = (sizeof(SourceParams) + 255) & ~(size_t)255;
   const size_t nsp = hs.spec_edges.size();   // table spectrum: edges and cumulative weights
   const size_t off_edges = bytes;
@@ -709,9 +755,16 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
       bytes += (sz[a] + 255) & ~(size_t)255;
     }
   }
-  if (!t->buf_src.ensure(bytes)) return set_error(DPM_E_NOMEM, "cannot allocate the source description");
+  if (fm) {
+    if (bytes > fm->src_bytes)
+      return set_error(DPM_E_INVALID, "source description of " + std::to_string(bytes) + " bytes exceeds the " +
+                                          std::to_string(fm->src_bytes) + " of the fixed workspace");
+  } else if (!t->buf_src.ensure(bytes)) {
+    return set_error(DPM_E_NOMEM, "cannot allocate the source description");
+  }
+  unsigned char* dst = fm ? fm->src : (unsigned char*)t->buf_src.p;
   std::vector<unsigned char> h(bytes, 0);
-  char* base = (char*)t->buf_src.p;
+  char* base = (char*)dst;
   SourceParams S;
   std::memset(&S, 0, sizeof(S));
   S.kind = hs.kind;
@@ -777,9 +830,11 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     D.sa = (const double*)(base + off[k][8]);
   }
   std::memcpy(h.data(), &S, sizeof(S));
-  // synchronous: the host staging vector goes out of scope on return
-  HIP_TRY(hipMemcpy(t->buf_src.p, h.data(), bytes, hipMemcpyHostToDevice));
-  *out = (SourceParams*)t->buf_src.p;
+  // ordered on the run stream, completed before the staging vector goes out
+  // of scope (the previous run on the stream may still read a fixed destination)
+  HIP_TRY(hipMemcpyAsync(dst, h.data(), bytes, hipMemcpyHostToDevice, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  *out = (SourceParams*)dst;
   return DPM_OK;
 }
 
@@ -799,7 +854,7 @@ int pipe_count(const dpm_run_options* opt) { return opt->pipes == 2 ? 2 : 1; }
 template <typename R>
 int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, const dpm_run_options* opt,
                   Workspace<R>* ws, Tables<R>& T, const ChunkPlan& plan, double* d_rows, double* d_steps,
-                  double* d_sum, double* d_sum2, double* d_hist_edep, dpm_stats* st) {
+                  double* d_sum, double* d_sum2, double* d_hist_edep, dpm_stats* st, const FixedMem* fm = nullptr) {
   const hipStream_t s = t->stream;
   KParams<R> P;
   std::memset(&P, 0, sizeof(P));
@@ -843,7 +898,7 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
   const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
   SourceParams* dS = nullptr;
   int rc = upload_source(t, hs, p->geom == GEOM_SLAB ? -0.5 * p->lbox : -0.5 * p->nz * p->lbox, nbatch,
-                         opt->hist_begin, nhist, &dS);
+                         opt->hist_begin, nhist, &dS, fm);
   if (rc != DPM_OK) return rc;
 
   // the chunks in history order: whole batches per chunk (at least one, as
@@ -1259,6 +1314,16 @@ int dpm_tables_create(const dpm_table_desc* desc, int device, dpm_tables** out) 
   }
   for (auto& e : t->ev)
     if (hipEventCreate(&e) != hipSuccess) { dpm_tables_destroy(t); return set_error(DPM_E_DEVICE, "event creation failed"); }
+  // the device address of this GPU's scratch copy (the device is current)
+  {
+    void* sp = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+      dpm_tables_destroy(t);
+      return set_error(DPM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    t->scratch_dev = (unsigned char*)sp;
+  }
   for (auto& pe : t->pev)
     for (auto& e : pe)
       if (hipEventCreate(&e) != hipSuccess) { dpm_tables_destroy(t); return set_error(DPM_E_DEVICE, "event creation failed"); }
@@ -1500,6 +1565,92 @@ int dpm_tally_fetch(dpm_tables* t, double* tally, double* second) {
   }
   HIP_TRY(hipStreamSynchronize(t->stream));
   return DPM_OK;
+}
+
+// Histories per chunk that the scratch queues hold for a grid of `blocks`
+// blocks: the run's queue capacity is 16 records per history of a chunk plus
+// the unfilled tails of the per-wave append blocks plus slack (run_transport).
+static int64_t noalloc_chunk(int blocks, bool fp64) {
+  const size_t qbytes = kScratchBytes - kScratchSrcBytes;
+  const int cap = fp64 ? Workspace<double>::capacity_of(qbytes) : Workspace<float>::capacity_of(qbytes);
+  const int64_t holes = (int64_t)3 * blocks * (256 / 64) * 64;
+  return ((int64_t)cap - holes - 65536) / 16;
+}
+
+int dpm_simulate_noalloc(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
+                         const dpm_run_options* opt_in, double* d_tally, double* d_steps, dpm_stats* st) {
+  if (!t || !p || !src || !d_tally) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (!t->scratch_dev) return set_error(DPM_E_DEVICE, "no device scratch on this tables handle");
+  if (nhist == 0) return set_error(DPM_E_INVALID, "zero histories");
+  if (nhist > 0xFFFFFFFFull) return set_error(DPM_E_INVALID, "at most 2^32-1 histories per call");
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  if (opt_in) opt = *opt_in;
+  if (opt.hist_begin + nhist > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (opt.nbatch > 1) return set_error(DPM_E_INVALID, "no batch statistics without allocation (nbatch must be 0 or 1)");
+  if (opt.hist_edep) return set_error(DPM_E_INVALID, "no per-history tally without allocation");
+  if (opt.pipes > 1) return set_error(DPM_E_INVALID, "one pipe without allocation");
+  if (opt.graph) return set_error(DPM_E_INVALID, "graph replay instantiates graphs: not without allocation");
+  const int rc0 = validate_source(t, p, src);
+  if (rc0 != DPM_OK) return rc0;
+  opt.nbatch = 0;
+  opt.pipes = 1;
+  // the chunk that the scratch queues hold at this grid (default: one block
+  // of 4 waves per CU, whose append-block tails leave room for the most histories)
+  const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 4;
+  opt.waves_per_cu = wpc;
+  const int blocks = std::max(1, t->num_cu * std::max(1, wpc / 4));
+  const int64_t cmax = noalloc_chunk(blocks, opt.precision == DPM_FP64);
+  if (cmax < 64) return set_error(DPM_E_INVALID, "waves_per_cu too large for the device scratch");
+  if (opt.chunk > cmax)
+    return set_error(DPM_E_INVALID, "chunk of " + std::to_string(opt.chunk) + " histories exceeds the " +
+                                        std::to_string(cmax) + " the device scratch holds at this grid");
+  const ChunkPlan plan{opt.chunk > 0 ? opt.chunk : cmax, 1};
+  HIP_TRY(hipSetDevice(t->device));
+  const hipStream_t s = t->stream;
+  const bool slab = p->geom == GEOM_SLAB;
+  const int64_t nb = slab ? p->nbins : p->nvox;
+  if (slab && !d_steps) return set_error(DPM_E_INVALID, "a slab run needs the steps array");
+  static_assert(sizeof(StatsDev) % sizeof(double) == 0, "the statistics block is cleared as doubles");
+  // the caller's outputs start at zero (they accumulate atomics)
+  const unsigned zb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nb + 255) / 256, (int64_t)t->num_cu * 8));
+  hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_tally, nb);
+  if (slab && d_steps) hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_steps, nb);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, (double*)t->stats, (int64_t)(sizeof(StatsDev) / sizeof(double)));
+  HIP_TRY(hipGetLastError());
+  // source description and queues in this GPU's scratch
+  FixedMem fm;
+  fm.src = t->scratch_dev;
+  fm.src_bytes = kScratchSrcBytes;
+  unsigned char* qbase = t->scratch_dev + kScratchSrcBytes;
+  const size_t qbytes = kScratchBytes - kScratchSrcBytes;
+  dpm_stats local;
+  std::memset(&local, 0, sizeof(local));
+  int rc;
+  const HostSource hs = source_from(src, nhist, opt.hist_begin);
+  if (opt.precision == DPM_FP64) {
+    Workspace<double> ws[kPipes];
+    ws[0].view(qbase, qbytes);
+    rc = run_transport<double>(t, p, hs, &opt, ws, t->td, plan, d_tally, slab ? d_steps : nullptr, nullptr, nullptr,
+                               nullptr, &local, &fm);
+  } else {
+    Workspace<float> ws[kPipes];
+    ws[0].view(qbase, qbytes);
+    rc = run_transport<float>(t, p, hs, &opt, ws, t->tf, plan, d_tally, slab ? d_steps : nullptr, nullptr, nullptr,
+                              nullptr, &local, &fm);
+  }
+  if (rc == DPM_OK) {
+    hipLaunchKernelGGL(k_sum, dim3(std::max(1, t->num_cu * 4)), dim3(256), 0, s, d_tally, nb, &t->stats->scored);
+    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "tally sum launch failed");
+  }
+  if (rc == DPM_OK) rc = finish_stats(t, &local);
+  // the result is the caller's: nothing stays behind for dpm_tally_fetch
+  t->last_tally = t->last_second = nullptr;
+  t->last_n = 0;
+  if (st) *st = local;
+  return rc;
 }
 
 int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet* b, int nb,

@@ -26,7 +26,8 @@ EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_t
             "dpm_phantom_create_slab", "dpm_phantom_create_voxel", "dpm_phantom_destroy", "dpm_phantom_num_bins",
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
             "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_simulate_fields", "dpm_last_error",
-            "dpm_abi_version", "dpm_probe_atomics", "dpm_phantom_set_majorant", "dpm_phantom_get_majorant")
+            "dpm_abi_version", "dpm_probe_atomics", "dpm_phantom_set_majorant", "dpm_phantom_get_majorant",
+            "dpm_simulate_noalloc")
 
 
 class DpmError(RuntimeError):
@@ -125,6 +126,8 @@ def lib():
         L.dpm_host_free.argtypes = [vp]
         L.dpm_probe_atomics.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp]
         L.dpm_phantom_set_majorant.argtypes = [vp, dp]
+        L.dpm_simulate_noalloc.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64,
+                                           ctypes.POINTER(RunOptions), vp, vp, ctypes.POINTER(Stats)]
         L.dpm_phantom_get_majorant.argtypes = [vp]
         L.dpm_phantom_get_majorant.restype = dp
         _lib = L
@@ -313,6 +316,22 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
         tally = tally.reshape(phantom.shape)
         steps = steps.reshape(phantom.shape)
     return tally, steps, st.as_dict(), he
+
+
+def simulate_noalloc(tables, phantom, nhist, d_tally, d_steps=None, particle=ELECTRON, energy=15.0, emin=0.0,
+                     emax=0.0, field=0.0, x0=0.0, y0=0.0, seed=1, hist_begin=0, precision=DPM_FP32, chunk=0,
+                     waves_per_cu=0, refill_min=0, tally_mode=0, angle=None, emode=None, spectrum_table=None):
+    """dpm_simulate_noalloc: the run without any device allocation, into
+    caller-owned device arrays of phantom.nbins doubles given as addresses
+    (d_tally; d_steps for the slab). Returns the stats dict."""
+    src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode, spectrum_table)
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
+                     refill_min=refill_min, tally_mode=tally_mode)
+    st = Stats()
+    _check(lib().dpm_simulate_noalloc(tables.h, phantom.h, ctypes.byref(src), nhist, ctypes.byref(opt),
+                                      ctypes.c_void_p(d_tally), None if d_steps is None else ctypes.c_void_p(d_steps),
+                                      ctypes.byref(st)))
+    return st.as_dict()
 
 
 def _source(particle, energy, emin, emax, field, x0, y0, angle, emode, table=None):

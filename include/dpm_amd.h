@@ -262,6 +262,27 @@ typedef struct dpm_field {
 int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, const dpm_field* fields, int nf,
                         const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
 
+/* No counterpart in the reference: the form of dpm_simulate for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc,
+ * hipFree or allocating library routine: the particle queues and the source
+ * description live in a static device array of 128 MiB per GPU (held from the
+ * library's first use on that GPU until the process ends), and the results go
+ * to caller-owned DEVICE arrays of dpm_phantom_num_bins doubles, which the
+ * call clears first: d_tally (energy per depth bin or voxel, MeV) and, for the
+ * slab, d_steps (scoring calls per bin; required there, ignored for voxels).
+ * Same kernels and per-history results as dpm_simulate. Limits, each an
+ * error: no batch statistics (nbatch <= 1), no per-history tally, one pipe,
+ * no graph replay, a spectrum table of at most ~4000 bins (64 KiB of source
+ * description), and queue chunks of at most what the scratch holds at the
+ * launch grid (waves_per_cu 0 means 4 here; chunk 0 takes the maximum: on
+ * 256 CUs about 31000 FP32 / 12700 FP64 histories at 4 waves per CU, 19000
+ * FP32 at 8; runs of any length are cut into such chunks). Every call on a GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (from several threads or tables handles) race. The call
+ * returns when the run is complete. */
+int dpm_simulate_noalloc(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
+                         const dpm_run_options* opt, double* d_tally, double* d_steps, dpm_stats* stats);
+
 /* Copies the tally of the last dpm_simulate on this handle (and, if
  * `second` is not NULL, its steps / sum-of-squares array) into host arrays of
  * dpm_phantom_num_bins elements. No counterpart in the reference (which writes
