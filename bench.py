@@ -81,6 +81,29 @@ def stream_copy_gbs(gib=1.0, reps=3):
     return 2.0 * n * 4 / best / 1e9
 
 
+# worker processes of the CPU baselines: every core this process may run on,
+# at most the 16 host cores that a one-GPU share of the box provides
+CPU_SHARE = 16
+
+
+def host_cpus():
+    """The host CPU the baselines run on: cores available to this process
+    (affinity), the OS count, physical cores and model from lscpu, and the
+    number of worker processes used."""
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict((a.strip(), b.strip()) for a, b in (l.split(":", 1) for l in out.splitlines() if ":" in l))
+        info["model"] = kv.get("Model name")
+        info["physical_cores"] = int(kv.get("Core(s) per socket", 0)) * int(kv.get("Socket(s)", 0)) or None
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", 0)) or None
+    except Exception as e:   # lscpu missing: the affinity count still stands
+        info["lscpu_error"] = str(e)
+    info["workers"] = min(CPU_SHARE, info["affinity_cpus"])
+    info["workers_rule"] = "min(%d, affinity CPUs): the host share of one GPU" % CPU_SHARE
+    return info
+
+
 def cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, seed, gpu_dose=None, gamma=None):
     """The FP64 oracle CLI on this host's cores (<= 16 processes) over exactly
     the histories of one timed step (seed, ids [0, nhist) split into disjoint
@@ -92,7 +115,8 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, seed, gpu_dose=Non
     cli = oracle_lib.CLI_PATH
     if not os.path.exists(cli):
         return None
-    cores = min(16, os.cpu_count() or 1)
+    host = host_cpus()
+    cores = host["workers"]
     args = [cli, "--data", tables0, "--mode", "philox", "--seed", str(seed), "--geom", "voxel", "--nx", str(nx),
             "--lbox", str(lbox), "--ptype", "0", "--emin", str(emin), "--emax", str(emax), "--field", str(field)]
     d = tempfile.mkdtemp(prefix="dpm_cpu_")
@@ -118,10 +142,16 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, seed, gpu_dose=Non
             if os.path.exists(f):
                 os.remove(f)
         os.rmdir(d)
-    r = {"value": nhist / wall, "unit": "histories/s", "cores": cores, "kind": "port",
+    # rate over the slowest process's transport time (each process reports
+    # the time of its oracle_run; start-up and table loading are excluded, as
+    # in the GPU's value, whose tables are resident before the timed region)
+    slowest = max(p["seconds"] for p in per)
+    r = {"value": nhist / slowest, "unit": "histories/s", "cores": cores, "kind": "port",
          "sample": "the timed step's %d histories (seed %d) in %d processes with disjoint history ranges (FP64 "
-                   "oracle, Philox mode), wall %.1f s; single-core %.0f hist/s"
-                   % (nhist, seed, cores, wall, sum(p["hist_per_s"] for p in per) / cores)}
+                   "oracle, Philox mode), slowest transport %.2f s; wall with process start-up and table loading "
+                   "%.2f s; single-core %.0f hist/s"
+                   % (nhist, seed, cores, slowest, wall, sum(p["hist_per_s"] for p in per) / cores),
+         "value_wall_inclusive": nhist / wall, "host": host}
     if cpu_dose is not None:
         ref = cpu_dose.reshape(gpu_dose.shape)
         m = ref >= 0.1 * ref.max()
@@ -136,7 +166,8 @@ def cpu_baseline(tables0, nx, lbox, field, emin, emax, nhist, seed, gpu_dose=Non
 
 def _oracle_voxel_worker(spec):
     """One CPU-baseline process (bench.py --oracle-worker JSON): the FP64
-    oracle on a phantom saved as .npy; prints its wall time. Touches no GPU."""
+    oracle on a phantom saved as .npy; prints its transport time and, when
+    asked, saves its dose. Touches no GPU."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # the checker, timed as the CPU baseline
@@ -145,24 +176,37 @@ def _oracle_voxel_worker(spec):
     rho = np.load(a["rho"], mmap_mode="r")
     oracle_lib.load(a["tables"])   # table loading stays outside the timed region
     t0 = time.perf_counter()
-    oracle_lib.run_voxel(a["tables"], mat, rho, a["lbox"], **a["kw"])
-    print(json.dumps({"seconds": time.perf_counter() - t0}), flush=True)
+    r = oracle_lib.run_voxel(a["tables"], mat, rho, a["lbox"], **a["kw"])
+    sec = time.perf_counter() - t0
+    if a.get("dose"):
+        np.save(a["dose"], r["edep"])
+    print(json.dumps({"seconds": sec}), flush=True)
 
 
-def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=4000):
-    """The FP64 oracle on this host's cores (<= 16 processes) over `jobs`, a
-    list of run_voxel keyword sets (history ranges of the workload); each
-    process runs about seconds_target."""
+def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=4000, keep_dose=True):
+    """The FP64 oracle on this host's cores over `jobs`, a list of run_voxel
+    keyword sets (sources of the workload with their first history id);
+    process i runs the (i // len(jobs))-th slice of n histories of job
+    i % len(jobs), each process about seconds_target. The majorant scale is
+    the one the GPU phantom uses, max(rho / rho_nominal). Returns the rate,
+    the slices run [(job, hist_begin, n)] and, with keep_dose, the summed
+    FP64 dose of all processes (for an equal-history comparison)."""
     import tempfile
     import numpy as np
-    cores = min(16, os.cpu_count() or 1)
-    d = tempfile.mkdtemp(prefix="dpm_cpu_")
+    import phantoms
+    host = host_cpus()
+    cores = host["workers"]
+    nominal = np.array([phantoms.NOMINAL_RHO[k] for k in sorted(phantoms.NOMINAL_RHO)])
+    scale = float(np.max(rho / nominal[mat]))
+    tmp = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    d = tempfile.mkdtemp(prefix="dpm_cpu_", dir=tmp)
     mp_, rp_ = os.path.join(d, "mat.npy"), os.path.join(d, "rho.npy")
     np.save(mp_, mat)
     np.save(rp_, rho)
 
-    def spawn(kw):
-        spec = json.dumps({"tables": tables, "mat": mp_, "rho": rp_, "lbox": lbox, "kw": kw})
+    def spawn(kw, dose=None):
+        spec = json.dumps({"tables": tables, "mat": mp_, "rho": rp_, "lbox": lbox,
+                           "kw": dict(kw, majorant_scale=scale), "dose": dose})
         return subprocess.Popen([sys.executable, os.path.abspath(__file__), "--oracle-worker", spec],
                                 stdout=subprocess.PIPE, text=True)
 
@@ -172,27 +216,50 @@ def cpu_baseline_voxel(tables, mat, rho, lbox, jobs, seconds_target=15.0, cal=40
             raise RuntimeError("oracle worker failed")
         return json.loads(out.strip().splitlines()[-1])["seconds"]
 
+    total = None
     try:
         sec = seconds(spawn(dict(jobs[0], nhist=cal)))
         n_each = max(cal, int(cal / sec * seconds_target))
-        # disjoint history ranges: process i takes the (i // len(jobs))-th slice of job i % len(jobs)
+        slices = [(i % len(jobs), jobs[i % len(jobs)].get("hist_begin", 0) + (i // len(jobs)) * n_each, n_each)
+                  for i in range(cores)]
+        files = [os.path.join(d, "dose%d.npy" % i) if keep_dose else None for i in range(cores)]
         t0 = time.perf_counter()
-        procs = [spawn(dict(jobs[i % len(jobs)], nhist=n_each,
-                            hist_begin=jobs[i % len(jobs)].get("hist_begin", 0) + (i // len(jobs)) * n_each))
-                 for i in range(cores)]
+        procs = [spawn(dict(jobs[j], nhist=n, hist_begin=hb), files[i]) for i, (j, hb, n) in enumerate(slices)]
         secs = [seconds(p) for p in procs]
         wall = time.perf_counter() - t0
-    finally:
-        for f in (mp_, rp_):
-            if os.path.exists(f):
+        if keep_dose:
+            for f in files:
+                part = np.load(f)
+                total = part if total is None else total + part
                 os.remove(f)
+    finally:
+        for f in os.listdir(d):
+            os.remove(os.path.join(d, f))
         os.rmdir(d)
     # rate over the slowest process's transport time (Python start-up and
-    # table loading excluded, so the CPU is not understated)
-    return {"value": cores * n_each / max(secs), "unit": "histories/s", "cores": cores, "kind": "port",
-            "sample": "%d concurrent processes x %d histories (FP64 oracle, Philox mode), slowest %.1f s "
-                      "(wall with start-up %.1f s); single-core %.0f hist/s"
-                      % (cores, n_each, max(secs), wall, n_each / (sum(secs) / cores))}
+    # table loading excluded, as for the GPU, whose tables are resident)
+    r = {"value": cores * n_each / max(secs), "unit": "histories/s", "cores": cores, "kind": "port",
+         "sample": "%d concurrent processes x %d histories (FP64 oracle, Philox mode), slowest %.1f s "
+                   "(wall with start-up %.1f s); single-core %.0f hist/s"
+                   % (cores, n_each, max(secs), wall, n_each / (sum(secs) / cores)),
+         "value_wall_inclusive": cores * n_each / wall, "host": host}
+    return r, slices, total
+
+
+def equal_history_report(gamma, cpu_dose, gpu_dose, rho, spacing):
+    """GPU FP32 dose against the CPU FP64 dose of the same histories: 1 %/1 mm
+    gamma over voxels above 10 % of the CPU maximum (dose = energy / density),
+    largest difference, total energy."""
+    import numpy as np
+    ref, ev = cpu_dose / rho, gpu_dose / rho
+    m = ref >= 0.1 * ref.max()
+    return {"gamma_pass_rate_gpu_fp32_vs_cpu": gamma.gamma_pass_rate_3d(ref, ev, spacing_mm=spacing, dd=0.01,
+                                                                         dta_mm=1.0, cutoff=0.1),
+            "voxels_compared": int(m.sum()),
+            "max_abs_diff_above_cutoff_rel_to_max": float(np.max(np.abs(ev[m] - ref[m])) / ref.max()),
+            "total_energy_rel_diff": float(abs(gpu_dose.sum() - cpu_dose.sum()) / cpu_dose.sum()),
+            "criterion": "1%/1mm global, voxels > 10% of the CPU dose maximum, full phantom; same histories and "
+                         "random streams"}
 
 
 def _read_hist_sim(path):
@@ -330,9 +397,16 @@ def config3_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
          "gamma_fp32_vs_fp64": {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max, full phantom",
                                 "fp64_histories_per_s": n / st64["seconds"]}}
     if cpu:
-        r["cpu_baseline"] = cpu_baseline_voxel(tables3, mat, rho, 2.0, [dict(ptype=-1, e0=12.0, field=100.0,
-                                                                              seed=3, hist_begin=10_000_000)])
-        r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
+        base, slices, cpu_dose = cpu_baseline_voxel(tables3, mat, rho, 2.0, [dict(ptype=-1, e0=12.0, field=100.0,
+                                                                                   seed=3, hist_begin=10_000_000)])
+        r["cpu_baseline"] = base
+        r["vs_cpu"] = r["histories_per_s"] / base["value"]
+        # the GPU's FP32 dose of exactly the CPU's histories (contiguous ids)
+        first, count = slices[0][1], sum(s[2] for s in slices)
+        g32, _, _, _ = dpm_amd.simulate(t, ph, count, particle=dpm_amd.ELECTRON, energy=12.0, field=100.0, seed=3,
+                                        hist_begin=first)
+        base["equal_history_dose"] = equal_history_report(gamma, cpu_dose, g32, rho, 2.0)
+        base["equal_history_dose"]["histories"] = count
     ph.close()
     t.close()
     return r
@@ -365,21 +439,31 @@ def config5_leg(dpm_amd, gamma, phantoms, tables3, cpu=True, fixed=None):
     sub = phantoms.config5_fields(1e8)
     a32, _, _ = dpm_amd.simulate_fields(t, ph, sub, **kw)
     a64, _, st64 = dpm_amd.simulate_fields(t, ph, sub, precision=dpm_amd.DPM_FP64, **kw)
-    c = (slice(64, 192), slice(192, 320), slice(192, 320))
-    pr = gamma.gamma_pass_rate_3d(a64[c] / rho[c], a32[c] / rho[c], spacing_mm=1.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
+    pr = gamma.gamma_pass_rate_3d(a64 / rho, a32 / rho, spacing_mm=1.0, dd=0.01, dta_mm=1.0, cutoff=0.1)
     r = {"workload": "config5: 512x512x256 heterogeneous phantom at 1 mm, 36 gantry photon fields every 10 deg, "
                      "10x10 cm2, triangular (0.1, 2, 18) MeV, uneven counts in [1e7, 5e7], 1e9 histories in one "
                      "run, FP32 transport, FP64 tally, 100 batches",
          "histories_per_s": n / wall, "seconds": wall, "schedule": sched,
          "gamma_fp32_vs_fp64": {"pass_rate": pr, "criterion": "1%/1mm global, voxels > 10% max, 1e8 histories "
-                                                              "of the plan, central 128^3 voxels",
+                                                              "of the plan, full phantom",
                                 "fp64_histories_per_s": sum(f[3] for f in sub) / st64["seconds"]}}
     if cpu:
         offs = np.concatenate([[0], np.cumsum([f[3] for f in plan])])
         jobs = [dict(ptype=0, emin=0.1, emode=2.0, emax=18.0, field=100.0, angle=plan[i][2], seed=5,
                      hist_begin=int(offs[i])) for i in range(len(plan))]
-        r["cpu_baseline"] = cpu_baseline_voxel(tables3, mat, rho, 1.0, jobs)
-        r["vs_cpu"] = r["histories_per_s"] / r["cpu_baseline"]["value"]
+        base, slices, cpu_dose = cpu_baseline_voxel(tables3, mat, rho, 1.0, jobs)
+        r["cpu_baseline"] = base
+        r["vs_cpu"] = r["histories_per_s"] / base["value"]
+        # the GPU's FP32 dose of exactly the CPU's histories: each process ran
+        # a slice of one gantry field (its own first history id)
+        g32 = np.zeros(ph.nbins)
+        for j, hb, cnt in slices:
+            d, _, _, _ = dpm_amd.simulate(t, ph, cnt, particle=dpm_amd.PHOTON, emin=0.1, emode=2.0, emax=18.0,
+                                          field=100.0, angle=plan[j][2], seed=5, hist_begin=hb)
+            g32 += d.ravel()
+        base["equal_history_dose"] = equal_history_report(gamma, cpu_dose, g32.reshape(mat.shape), rho, 1.0)
+        base["equal_history_dose"]["histories"] = int(sum(s[2] for s in slices))
+        base["equal_history_dose"]["fields"] = sorted(set(int(plan[j][2]) for j, _, _ in slices))
     ph.close()
     t.close()
     return r
@@ -408,8 +492,11 @@ def voxel_error_report(d32, sum2, d64, nbatch, cutoff=0.1):
 def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, sched=None, plain=True):
     """Config-4 workload: beamlets on the water phantom, plain (one dpm_simulate
     per beamlet) vs batched (one dpm_simulate_beamlets call). Both FP32 with the
-    same history ids; only GPU calls are timed (no host thresholding for the
-    plain form, which favours it)."""
+    same history ids. The plain form is timed three ways: with the default
+    and with its own tuned schedule, each leaving its result on the device (no
+    copy, no thresholding), and with the default schedule copying each dense
+    result to pinned host memory. The headline ratio takes the fastest plain
+    form."""
     import numpy as np
     import phantoms
     bl = phantoms.beamlets(n_beamlets)
@@ -424,13 +511,26 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
     t_batched = time.perf_counter() - t0
     out = (dpm_amd.pinned_empty(ph.nbins), dpm_amd.pinned_empty(ph.nbins))
     offs = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
-    t_plain = None
+    t_plain = t_plain_fetch = plain_sched = None
+    plain_times = {}
     if plain:
+        # the plain form's schedule: tuned on the median beamlet (one pencil beam per run)
+        mid = sorted(bl, key=lambda b: b[4])[len(bl) // 2]
+        plain_sched = dpm_amd.autotune(t, ph, nsample=max(1000, int(mid[4])), particle=mid[2], energy=mid[3],
+                                       x0=mid[0], y0=mid[1])
+        plain_sched["source"] = "dpm_autotune on the median beamlet (%d histories)" % mid[4]
+        for name, sch in (("default_schedule", None), ("tuned_schedule", plain_sched)):
+            t0 = time.perf_counter()
+            for i, (x, y, p, e, n) in enumerate(bl):
+                dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed, hist_begin=int(offs[i]),
+                                 schedule=sch, fetch=False)
+            plain_times[name] = time.perf_counter() - t0
+        t_plain = min(plain_times.values())
         t0 = time.perf_counter()
         for i, (x, y, p, e, n) in enumerate(bl):
             dpm_amd.simulate(t, ph, n, particle=p, energy=e, x0=x, y0=y, seed=seed, hist_begin=int(offs[i]),
                              out=out)
-        t_plain = time.perf_counter() - t0
+        t_plain_fetch = time.perf_counter() - t0
     # correctness: batched FP32 rows of the largest beamlets vs FP64 plain runs
     order = np.argsort([-b[4] for b in bl])[:n_check]
     rates = []
@@ -449,7 +549,16 @@ def beamlet_leg(dpm_amd, gamma, t, ph, nx, lbox, n_beamlets, seed=7, n_check=4, 
             "speedup": t_plain / t_batched if t_plain else None,
             "batched_histories_per_s": total / t_batched,
             "plain_histories_per_s": total / t_plain if t_plain else None,
-            "nnz": int(row_ptr[-1]), "batched_schedule": sched,
+            "plain_what": "one dpm_simulate per beamlet, result left on the device: the faster of the default and "
+                          "the tuned schedule",
+            "plain_s_by_schedule": plain_times,
+            "plain_fetch_s": t_plain_fetch,
+            "speedup_vs_plain_fetch": t_plain_fetch / t_batched if t_plain_fetch else None,
+            "plain_fetch_what": "one dpm_simulate per beamlet, default schedule, dense result copied to pinned host "
+                                "memory",
+            "batched_what": "one dpm_simulate_beamlets call and dpm_beamlets_fetch: transport, CSR build, CSR "
+                            "copied to host memory",
+            "nnz": int(row_ptr[-1]), "batched_schedule": sched, "plain_schedule": plain_sched,
             "gamma_batched_fp32_vs_plain_fp64": {"beamlets": [int(i) for i in order], "pass_rates": rates,
                                                  "criterion": "1%/1mm global, voxels > 10% of the beamlet max"}}
 
@@ -558,11 +667,16 @@ def main():
     bytes_per_launch = algorithmic_bytes(cnt) / max(1, launches)
     avg_ms = ms_dom / max(1, launches)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM traffic per launch is not measured in this run: it comes from the
+    # committed rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of this
+    # workload, named in roofline.traffic_source
+    traffic = traffic_source = None
     prof = os.path.join(ROOT, "profiles", "traffic_k_electrons_water150_photons.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tr = json.load(f)
+        traffic = tr.get("bytes_per_launch")
+        traffic_source = tr.get("source", "profiles/traffic_k_electrons_water150_photons.json")
     atomics = atomic_roofline(dpm_amd, t, st, ph.nbins * max(1, args.nbatch))
 
     result = {
@@ -574,7 +688,8 @@ def main():
                                % (nhist, args.nbatch),
                    "histories": nhist, "voxels": nx ** 3, "nbatch": args.nbatch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
+                     "kernel": "k_" + dom,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": launches,
                      # with two pipes, launches of the two streams share the GPU: the
                      # per-launch rate above counts each launch over its whole duration;

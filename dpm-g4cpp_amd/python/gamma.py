@@ -24,19 +24,30 @@ def _offsets(ndim, spacing_mm, dta_mm, sub):
 
 
 def gamma_map(ref, ev, spacing_mm, dd=0.01, dta_mm=1.0, cutoff=0.1, sub=4):
+    """Gamma at every reference point above the cutoff (NaN elsewhere) and the
+    mask of those points. The evaluated distribution is interpolated
+    (trilinear, edge values extended) at r + o for each search offset o, at the
+    masked points only."""
     ref = np.asarray(ref, dtype=np.float64)
     ev = np.asarray(ev, dtype=np.float64)
     dmax = ref.max()
     if dmax <= 0:
         raise ValueError("reference distribution is empty")
     off, dist = _offsets(ref.ndim, spacing_mm, dta_mm, sub)
-    best = np.full(ref.shape, np.inf)
-    for o, d in zip(off, dist):
-        shifted = ndimage.shift(ev, -o / spacing_mm, order=1, mode="nearest") if np.any(o) else ev
-        g2 = ((shifted - ref) / (dd * dmax)) ** 2 + (d / dta_mm) ** 2
-        np.minimum(best, g2, out=best)
     mask = ref >= cutoff * dmax
-    return np.sqrt(best), mask
+    pts = np.array(np.nonzero(mask), dtype=np.float64)
+    r = ref[mask]
+    best = np.full(r.shape, np.inf)
+    for o, d in zip(off, dist):
+        if np.any(o):
+            val = ndimage.map_coordinates(ev, pts + (o / spacing_mm)[:, None], order=1, mode="nearest")
+        else:
+            val = ev[mask]
+        g2 = ((val - r) / (dd * dmax)) ** 2 + (d / dta_mm) ** 2
+        np.minimum(best, g2, out=best)
+    g = np.full(ref.shape, np.nan)
+    g[mask] = np.sqrt(best)
+    return g, mask
 
 
 def gamma_pass_rate(ref, ev, spacing_mm, dd=0.01, dta_mm=1.0, cutoff=0.1, sub=4):

@@ -50,3 +50,24 @@ def test_read_hist_sim_reference_fixture(bench):
     assert rows.shape[1] == 3 and rows.shape[0] >= 100
     np.testing.assert_allclose(rows[:, 0], 0.05 + 0.1 * np.arange(rows.shape[0]), rtol=1e-4)
     assert rows[:, 1].max() > 0 and np.all(rows[:, 2] >= 0)
+
+
+def test_host_cpus_records_the_cores_used(bench):
+    h = bench.host_cpus()
+    assert h["affinity_cpus"] == len(os.sched_getaffinity(0))
+    assert 1 <= h["workers"] <= min(bench.CPU_SHARE, h["affinity_cpus"])
+    assert "physical_cores" in h or "lscpu_error" in h
+
+
+def test_equal_history_report(bench):
+    import gamma
+    rng = np.random.default_rng(3)
+    d = rng.uniform(0.0, 1.0, size=(10, 12, 14))
+    rho = np.full(d.shape, 2.0)
+    r = bench.equal_history_report(gamma, d, d.copy(), rho, 2.0)
+    assert r["gamma_pass_rate_gpu_fp32_vs_cpu"] == 1.0 and r["max_abs_diff_above_cutoff_rel_to_max"] == 0.0
+    assert r["voxels_compared"] == int((d >= 0.1 * d.max()).sum())
+    # a 3 % error everywhere fails every voxel at 1 %/1 mm on 2 mm voxels of noise
+    bad = bench.equal_history_report(gamma, d, d * 1.03, rho, 2.0)
+    assert bad["gamma_pass_rate_gpu_fp32_vs_cpu"] < 0.5
+    assert abs(bad["total_energy_rel_diff"] - 0.03) < 1e-12
