@@ -45,39 +45,6 @@ int set_error(int code, const std::string& msg) {
       return set_error(DPM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));           \
   } while (0)
 
-// ---------------------------------------------------------------------------
-// natural cubic spline coefficients, same recurrences and operation order as
-// SimDataSpline::SetUp (SimDataSpline.cc:33-67)
-// ---------------------------------------------------------------------------
-void spline_coefficients(const double* x, const double* y, int n, std::vector<double>& B, std::vector<double>& C,
-                         std::vector<double>& D) {
-  B.assign(n, 0.0); C.assign(n, 0.0); D.assign(n, 0.0);
-  const int last = n - 1;
-  double s = 0.0, r = 0.0;
-  for (int m = 0; m < last; ++m) {
-    D[m] = x[m + 1] - x[m];
-    r = (y[m + 1] - y[m]) / D[m];
-    C[m] = r - s;
-    s = r;
-  }
-  r = 0.0; s = 0.0;
-  C[0] = 0.0; C[last] = 0.0;
-  for (int m = 1; m < last; ++m) {
-    C[m] = C[m] + r * C[m - 1];
-    B[m] = 2.0 * (x[m - 1] - x[m + 1]) - r * s;
-    s = D[m];
-    r = s / B[m];
-  }
-  for (int m = last - 1; m >= 1; --m) C[m] = (D[m] * C[m + 1] - C[m]) / B[m];
-  for (int m = 0; m < last; ++m) {
-    s = D[m];
-    r = C[m + 1] - C[m];
-    D[m] = r / s;
-    C[m] = 3.0 * C[m];
-    B[m] = (y[m + 1] - y[m]) / s - (C[m] + r) * s;
-  }
-}
-
 // integer keep-threshold of an alias entry: (w < f * 2^-24) <=> (f >= thr)
 uint32_t alias_threshold(double w) {
   if (!(w >= 0.0)) return 0u;
@@ -110,7 +77,7 @@ struct PackBuilder {
     std::vector<SplineNode<R>> nodes((size_t)nm * n);
     for (int im = 0; im < nm; ++im) {
       std::vector<double> B, C, D;
-      spline_coefficients(per[im].x, per[im].y, n, B, C, D);
+      SimSplineCoefficients(per[im].x, per[im].y, n, B, C, D);
       for (int i = 0; i < n; ++i) {
         SplineNode<R>& nd = nodes[(size_t)im * n + i];
         nd.x = (R)per[im].x[i]; nd.y = (R)per[im].y[i];

@@ -1,12 +1,14 @@
 // SimData.cc -- readers of the reference's run-time table files.
 //
-// Each Load follows the corresponding reference reader's layout exactly
+// Each reader follows the corresponding reference reader's layout exactly
 // (file name, number of comment lines skipped, columns used); see the
-// citations per function. Errors are reported as text instead of the
-// reference's exit(EXIT_FAILURE).
+// citations per function. Load(dir, err) reports an error as text;
+// Load(dir[, verbose]) prints it and ends the process with EXIT_FAILURE, as
+// the reference's loaders do.
 #include "SimData.hh"
 
 #include <cstdio>
+#include <cstdlib>
 #include <iostream>
 
 namespace {
@@ -121,7 +123,122 @@ bool readEnergyAlias(const std::string& path, SimAliasSet& s, std::string& err) 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// readers of the single files
+// ---------------------------------------------------------------------------
+// SimITr1MFPElastic.cc:13-56 -- column 3 (scattering-power corrected)
+bool SimReadITr1MFP(const std::string& dataDir, std::vector<SimNodes>& out, std::string& err) {
+  return readPerMaterial(dataDir + "/el_itr1mfp.dat", 4, 3, 2, out, err);
+}
+// SimMaxScatStrength.cc:6-43
+bool SimReadMaxScatStrength(const std::string& dataDir, SimNodes& out, std::string& err) {
+  return readReference(dataDir + "/el_scatStrength.dat", 4, out, err);
+}
+// SimIMFPMoller.cc:6-43
+bool SimReadIMFPMoller(const std::string& dataDir, SimNodes& out, std::string& err) {
+  return readReference(dataDir + "/imfp_moller.dat", 3, out, err);
+}
+// SimIMFPBrem.cc:18-61
+bool SimReadIMFPBrem(const std::string& dataDir, std::vector<SimNodes>& out, std::string& err) {
+  return readPerMaterial(dataDir + "/imfp_brem.dat", 3, 2, 1, out, err);
+}
+// SimStoppingPower.cc:18-61 (two fused comment lines in the file)
+bool SimReadDEDX(const std::string& dataDir, std::vector<SimNodes>& out, std::string& err) {
+  return readPerMaterial(dataDir + "/eloss_rdedx.dat", 2, 2, 1, out, err);
+}
+// SimMollerTables.cc:20-67
+bool SimReadMollerTables(const std::string& dataDir, SimAliasSet& out, std::string& err) {
+  return readEnergyAlias(dataDir + "/ioni_MollerDtrData.dat", out, err);
+}
+// SimSBTables.cc:21-90
+bool SimReadSBTables(const std::string& dataDir, SimAliasSet& s, std::string& err) {
+  const std::string path = dataDir + "/brem_SBDtrData.dat";
+  TableFile tf(path);
+  if (!tf.ok()) return fail(err, "file = " + path + " not found!");
+  tf.skipLines(5);
+  if (!tf.num(s.ne) || !tf.num(s.ns) || !tf.num(s.nmat) || s.ne < 2 || s.ns < 2 || s.nmat < 1)
+    return fail(err, "malformed " + path);
+  sizeAlias(s);
+  for (int ie = 0; ie < s.ne; ++ie)
+    if (!tf.num(s.energy[ie])) return fail(err, "malformed " + path);
+  tf.eatSpace();
+  for (int im = 0; im < s.nmat; ++im) {
+    tf.skipLines(3);
+    for (int ie = 0; ie < s.ne; ++ie)
+      if (!readAliasRows(tf, s, ((size_t)im * s.ne + ie) * s.ns)) return fail(err, "malformed " + path);
+    tf.eatSpace();
+  }
+  return true;
+}
+// SimGSTables.cc:19-80 (reference material only)
+bool SimReadGSTables(const std::string& dataDir, int& ne, int& ns, std::vector<double>& energy,
+                     std::vector<double>& transf, std::vector<double>& u, std::vector<double>& a,
+                     std::vector<double>& b, std::string& err) {
+  const std::string path = dataDir + "/el_GSDtrData.dat";
+  TableFile tf(path);
+  if (!tf.ok()) return fail(err, "file = " + path + " not found!");
+  tf.skipLines(4);
+  double emin = 0, emax = 0;
+  if (!tf.num(ne) || !tf.num(emin) || !tf.num(emax) || !tf.num(ns) || ne < 2 || ns < 2)
+    return fail(err, "malformed " + path);
+  tf.eatSpace();
+  tf.skipLines(6);
+  const size_t n = (size_t)ne * ns;
+  energy.resize(ne);
+  transf.resize(ne);
+  u.resize(n); a.resize(n); b.resize(n);
+  for (int ie = 0; ie < ne; ++ie) {
+    int idx = 0;
+    if (!tf.num(idx) || !tf.num(energy[ie]) || !tf.num(transf[ie])) return fail(err, "malformed " + path);
+    for (int is = 0; is < ns; ++is) {
+      const size_t k = (size_t)ie * ns + is;
+      if (!tf.num(u[k]) || !tf.num(a[k]) || !tf.num(b[k])) return fail(err, "malformed " + path);
+    }
+  }
+  return true;
+}
+// SimIMFPMaxPhoton.cc:6-38: 3 comment lines, n, 1 comment line, n rows
+bool SimReadIMFPMax(const std::string& dataDir, SimNodes& out, std::string& err) {
+  const std::string path = dataDir + "/imfp_globalMax.dat";
+  TableFile tf(path);
+  if (!tf.ok()) return fail(err, "file = " + path + " not found!");
+  tf.skipLines(3);
+  int n = 0;
+  if (!tf.num(n) || n < 2) return fail(err, "malformed " + path);
+  tf.eatSpace();
+  tf.skipLines(1);
+  out.x.resize(n);
+  out.y.resize(n);
+  for (int i = 0; i < n; ++i)
+    if (!tf.num(out.x[i]) || !tf.num(out.y[i])) return fail(err, "malformed " + path);
+  return true;
+}
+// SimIMFPPhoton.cc:19-63 (type 0 total, 1 Compton, 2 pair production)
+bool SimReadIMFPPhoton(const std::string& dataDir, int type, std::vector<SimNodes>& out, std::string& err) {
+  static const char* kFile[3] = {"/imfp_total.dat", "/imfp_compton.dat", "/imfp_pairp.dat"};
+  if (type < 0 || type > 2) return fail(err, "unknown photon IMFP type");
+  return readPerMaterial(dataDir + kFile[type], 3, 2, 1, out, err);
+}
+// SimKNTables.cc:20-67
+bool SimReadKNTables(const std::string& dataDir, SimAliasSet& out, std::string& err) {
+  return readEnergyAlias(dataDir + "/compton_KNDtrData.dat", out, err);
+}
+
+namespace {
+// the reference's loaders print the reason and end the process
+void LoadOrExit(bool ok, const std::string& err) {
+  if (ok) return;
+  std::cerr << "  *** ERROR: " << err << std::endl;
+  std::exit(EXIT_FAILURE);
+}
+}  // namespace
+
 // SimMaterialData::Load (SimMaterialData.cc:6-53)
+void SimMaterialData::Load(const std::string& dataDir, int verbose) {
+  std::string err;
+  LoadOrExit(Load(dataDir, err, verbose), err);
+}
+
 bool SimMaterialData::Load(const std::string& dataDir, std::string& err, int verbose) {
   const std::string path = dataDir + "/mat.dat";
   TableFile tf(path);
@@ -155,91 +272,46 @@ bool SimMaterialData::Load(const std::string& dataDir, std::string& err, int ver
 }
 
 // SimElectronData::Load (SimElectronData.cc:44-115) and the loaders it calls
+void SimElectronData::Load(const std::string& dataDir, int verbose) {
+  std::string err;
+  LoadOrExit(Load(dataDir, err, verbose), err);
+}
+
 bool SimElectronData::Load(const std::string& dataDir, std::string& err, int verbose) {
-  // SimITr1MFPElastic.cc:13-56 -- column 3 (scattering-power corrected)
-  if (!readPerMaterial(dataDir + "/el_itr1mfp.dat", 4, 3, 2, fITr1MFP, err)) return false;
-  // SimMaxScatStrength.cc:6-43
-  if (!readReference(dataDir + "/el_scatStrength.dat", 4, fMaxScatStrength, err)) return false;
-  // SimIMFPMoller.cc:6-43
-  if (!readReference(dataDir + "/imfp_moller.dat", 3, fIMFPMoller, err)) return false;
-  // SimIMFPBrem.cc:18-61
-  if (!readPerMaterial(dataDir + "/imfp_brem.dat", 3, 2, 1, fIMFPBrem, err)) return false;
-  // SimStoppingPower.cc:18-61 (two fused comment lines in the file)
-  if (!readPerMaterial(dataDir + "/eloss_rdedx.dat", 2, 2, 1, fDEDX, err)) return false;
-  // SimMollerTables.cc:20-67
-  if (!readEnergyAlias(dataDir + "/ioni_MollerDtrData.dat", fMollerTables, err)) return false;
-  // SimSBTables.cc:21-90
-  {
-    const std::string path = dataDir + "/brem_SBDtrData.dat";
-    TableFile tf(path);
-    if (!tf.ok()) return fail(err, "file = " + path + " not found!");
-    tf.skipLines(5);
-    SimAliasSet& s = fSBTables;
-    if (!tf.num(s.ne) || !tf.num(s.ns) || !tf.num(s.nmat) || s.ne < 2 || s.ns < 2 || s.nmat < 1)
-      return fail(err, "malformed " + path);
-    sizeAlias(s);
-    for (int ie = 0; ie < s.ne; ++ie)
-      if (!tf.num(s.energy[ie])) return fail(err, "malformed " + path);
-    tf.eatSpace();
-    for (int im = 0; im < s.nmat; ++im) {
-      tf.skipLines(3);
-      for (int ie = 0; ie < s.ne; ++ie)
-        if (!readAliasRows(tf, s, ((size_t)im * s.ne + ie) * s.ns)) return fail(err, "malformed " + path);
-      tf.eatSpace();
-    }
-  }
-  // SimGSTables.cc:19-80 (reference material only)
-  {
-    const std::string path = dataDir + "/el_GSDtrData.dat";
-    TableFile tf(path);
-    if (!tf.ok()) return fail(err, "file = " + path + " not found!");
-    tf.skipLines(4);
-    double emin = 0, emax = 0;
-    if (!tf.num(fGSNumEnergies) || !tf.num(emin) || !tf.num(emax) || !tf.num(fGSNumCum) || fGSNumEnergies < 2 ||
-        fGSNumCum < 2)
-      return fail(err, "malformed " + path);
-    tf.eatSpace();
-    tf.skipLines(6);
-    const size_t n = (size_t)fGSNumEnergies * fGSNumCum;
-    fGSEnergy.resize(fGSNumEnergies);
-    fGSTransf.resize(fGSNumEnergies);
-    fGSU.resize(n); fGSA.resize(n); fGSB.resize(n);
-    for (int ie = 0; ie < fGSNumEnergies; ++ie) {
-      int idx = 0;
-      if (!tf.num(idx) || !tf.num(fGSEnergy[ie]) || !tf.num(fGSTransf[ie])) return fail(err, "malformed " + path);
-      for (int is = 0; is < fGSNumCum; ++is) {
-        const size_t k = (size_t)ie * fGSNumCum + is;
-        if (!tf.num(fGSU[k]) || !tf.num(fGSA[k]) || !tf.num(fGSB[k])) return fail(err, "malformed " + path);
-      }
-    }
-  }
+  if (!SimReadITr1MFP(dataDir, fITr1MFP, err) || !SimReadMaxScatStrength(dataDir, fMaxScatStrength, err) ||
+      !SimReadIMFPMoller(dataDir, fIMFPMoller, err) || !SimReadIMFPBrem(dataDir, fIMFPBrem, err) ||
+      !SimReadDEDX(dataDir, fDEDX, err) || !SimReadMollerTables(dataDir, fMollerTables, err) ||
+      !SimReadSBTables(dataDir, fSBTables, err) ||
+      !SimReadGSTables(dataDir, fGSNumEnergies, fGSNumCum, fGSEnergy, fGSTransf, fGSU, fGSA, fGSB, err))
+    return false;
+  fITr1MFPElasticTab.Set(fITr1MFP);
+  fMaxScatStrengthTab.Set({fMaxScatStrength});
+  fIMFPMollerTab.Set({fIMFPMoller});
+  fIMFPBremTab.Set(fIMFPBrem);
+  fDEDXTab.Set(fDEDX);
+  fMollerTablesTab.Set(fMollerTables);
+  fSBTablesTab.Set(fSBTables);
+  fGSTablesTab.Set(fGSNumEnergies, fGSNumCum, fGSEnergy, fGSTransf, fGSU, fGSA, fGSB);
   if (verbose > 0) std::cout << " == Electron data loaded from " << dataDir << std::endl;
   return true;
 }
 
 // SimPhotonData::Load (SimPhotonData.cc:28-78)
+void SimPhotonData::Load(const std::string& dataDir, int verbose) {
+  std::string err;
+  LoadOrExit(Load(dataDir, err, verbose), err);
+}
+
 bool SimPhotonData::Load(const std::string& dataDir, std::string& err, int verbose) {
-  // SimIMFPMaxPhoton.cc:6-38: 3 comment lines, n, 1 comment line, n rows
-  {
-    const std::string path = dataDir + "/imfp_globalMax.dat";
-    TableFile tf(path);
-    if (!tf.ok()) return fail(err, "file = " + path + " not found!");
-    tf.skipLines(3);
-    int n = 0;
-    if (!tf.num(n) || n < 2) return fail(err, "malformed " + path);
-    tf.eatSpace();
-    tf.skipLines(1);
-    fIMFPTotalMax.x.resize(n);
-    fIMFPTotalMax.y.resize(n);
-    for (int i = 0; i < n; ++i)
-      if (!tf.num(fIMFPTotalMax.x[i]) || !tf.num(fIMFPTotalMax.y[i])) return fail(err, "malformed " + path);
-  }
-  // SimIMFPPhoton.cc:19-63 (types 0, 1, 2)
-  if (!readPerMaterial(dataDir + "/imfp_total.dat", 3, 2, 1, fIMFPTotal, err)) return false;
-  if (!readPerMaterial(dataDir + "/imfp_compton.dat", 3, 2, 1, fIMFPCompton, err)) return false;
-  if (!readPerMaterial(dataDir + "/imfp_pairp.dat", 3, 2, 1, fIMFPPairProd, err)) return false;
-  // SimKNTables.cc:20-67
-  if (!readEnergyAlias(dataDir + "/compton_KNDtrData.dat", fKNTables, err)) return false;
+  if (!SimReadIMFPMax(dataDir, fIMFPTotalMax, err) || !SimReadIMFPPhoton(dataDir, 0, fIMFPTotal, err) ||
+      !SimReadIMFPPhoton(dataDir, 1, fIMFPCompton, err) || !SimReadIMFPPhoton(dataDir, 2, fIMFPPairProd, err) ||
+      !SimReadKNTables(dataDir, fKNTables, err))
+    return false;
+  fIMFPTotalMaxTab.Set({fIMFPTotalMax});
+  fIMFPTotalTab.Set(fIMFPTotal);
+  fIMFPComptonTab.Set(fIMFPCompton);
+  fIMFPPairProdTab.Set(fIMFPPairProd);
+  fKNTablesTab.Set(fKNTables);
   if (verbose > 0) std::cout << " == Photon data loaded from " << dataDir << std::endl;
   return true;
 }

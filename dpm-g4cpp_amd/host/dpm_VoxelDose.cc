@@ -104,11 +104,21 @@ static bool ReadSpectrum(const std::string& path, std::vector<double>& edges, st
     return false;
   }
   std::string line;
+  int lineno = 0;
   while (std::getline(f, line)) {
-    if (line.empty() || line[0] == '#') continue;
+    ++lineno;
+    const size_t first = line.find_first_not_of(" \t\r");
+    if (first == std::string::npos || line[first] == '#') continue;   // blank or comment
+    // every other line is exactly three numbers: a typo or a truncated line
+    // is an error, not a silently dropped bin
     std::istringstream is(line);
     double lo, hi, w;
-    if (!(is >> lo >> hi >> w)) continue;
+    std::string rest;
+    if (!(is >> lo >> hi >> w) || (is >> rest)) {
+      std::cerr << " *** " << path << ":" << lineno << ": expected 'E_low E_high weight', got '" << line << "'"
+                << std::endl;
+      return false;
+    }
     if (edges.empty()) edges.push_back(lo);
     else if (lo != edges.back()) {
       std::cerr << " *** spectrum bins must be contiguous (" << path << ": " << lo << ")" << std::endl;
@@ -171,6 +181,9 @@ int main(int argc, char* argv[]) {
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
 
   dpm_tables* t = nullptr;
+  // input files are checked before the device is touched
+  std::vector<double> sp_edges, sp_weights;
+  if (!a.spectrum.empty() && !ReadSpectrum(a.spectrum, sp_edges, sp_weights)) return EXIT_FAILURE;
   Check(dpm_tables_load(a.data.c_str(), a.device, &t), "loading tables");
   std::vector<uint8_t> mat = a.mat_file.empty() ? std::vector<uint8_t>(nvox, 0) : ReadRaw<uint8_t>(a.mat_file, nvox);
   std::vector<float> rho;
@@ -239,9 +252,7 @@ int main(int argc, char* argv[]) {
     src.angle_deg = a.angle;
     src.spectrum = a.triangular ? DPM_SPEC_TRIANGULAR : DPM_SPEC_FLAT;
     src.emode = a.emode;
-    std::vector<double> sp_edges, sp_weights;
     if (!a.spectrum.empty()) {
-      if (!ReadSpectrum(a.spectrum, sp_edges, sp_weights)) return EXIT_FAILURE;
       src.spectrum = DPM_SPEC_TABLE;
       src.spec_n = (int)sp_weights.size();
       src.spec_edges = sp_edges.data();

@@ -115,3 +115,16 @@ def test_dose_file_readers_round_trip(tmp_path):
     q.write_bytes(b"# dpm-amd csr v1\n2 60 3\n" + rp.tobytes() + col.tobytes() + val.tobytes())
     r2, c2, v2, nvox = dpm_amd.read_csr(str(q))
     assert nvox == 60 and np.array_equal(r2, rp) and np.array_equal(c2, col) and np.array_equal(v2, val)
+
+
+@pytest.mark.parametrize("text", ["0.5 1.0 1.0\n1.0 2.0 0,5\n", "0.5 1.0 1.0\n1.0 2.0\n", "0.5 1.0 1.0 7\n",
+                                  "0.5 1.0 1.0\n1.0 2.0 x\n"])
+def test_voxeldose_rejects_a_malformed_spectrum_line(tmp_path, tables0, text):
+    # every non-blank, non-comment line must be exactly three numbers; the
+    # file is checked before the device is touched, so this runs without a GPU
+    spec = tmp_path / "s.txt"
+    spec.write_text("# E_low E_high weight\n\n" + text)
+    r = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", "4", "-y", "4", "-z", "4", "-S", str(spec),
+                        "-n", "10", "-o", str(tmp_path / "x.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "expected 'E_low E_high weight'" in r.stderr

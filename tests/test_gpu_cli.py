@@ -49,6 +49,29 @@ def test_dpm_simulate_cli_writes_reference_hist_sim(tables3, tmp_path):
     np.testing.assert_allclose(rows[m, 2], ref_vals[m, 2], rtol=2e-3)
 
 
+REFMAIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "dpm_Simulate_refmain")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(REFMAIN), reason="the reference's main is built only where its tree exists")
+def test_reference_main_unchanged_runs_on_the_engine(tables3, tmp_path):
+    # the reference's own dpm_Simulate.cc, compiled unchanged against the host
+    # data layer (oracle/Makefile target refmain): its Simulate() call runs the
+    # GPU engine with the defaults (FP32, Philox seed 1, device 0), so its
+    # hist.sim equals the engine's own command line with those settings
+    args = ["-p", "e-", "-e", "15", "-n", "20000", "-d", tables3, "-b", "1.0", "-c", "3"]
+    a, b = tmp_path / "ref_main.sim", tmp_path / "engine.sim"
+    subprocess.run([REFMAIN] + args + ["-o", str(a)], check=True, capture_output=True, timeout=120)
+    subprocess.run([dpm_amd.CLI_PATH] + args + ["-s", "1", "-o", str(b)], check=True, capture_output=True,
+                   timeout=120)
+    ha, ra = read_hist_sim(a)
+    hb, rb = read_hist_sim(b)
+    assert ha == hb and ra.shape == rb.shape
+    np.testing.assert_allclose(ra, rb, rtol=1e-9, atol=1e-12)
+    assert 13.0 < float(open(a).readline().split(":")[1].split()[0]) < 14.5
+
+
 @pytest.mark.gpu
 def test_dpm_voxeldose_cli_dose_and_uncertainty(gpu_tables0, tables0, tmp_path):
     nx, ny, nz, lbox, n, nb = 16, 14, 12, 2.0, 20000, 8
