@@ -302,6 +302,10 @@ struct Workspace {
   }
 };
 
+// device bytes for the dense beamlet rows of one group when the caller sets
+// no budget (dpm_run_options.tally_budget)
+constexpr size_t kBeamletBudget = (size_t)16 << 30;
+
 // most transport rounds enqueued per host synchronisation (and per captured
 // graph); rounds past the drain find empty queues and exit at once
 constexpr int kRoundsPerSync = 8;
@@ -1631,18 +1635,24 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   if (opt_in) opt = *opt_in;
   if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
   if (opt.pipes < 0 || opt.pipes > kPipes) return set_error(DPM_E_INVALID, "pipes must be 0, 1 or 2");
-  // one source entry per beamlet; its histories keep the ids they have when
-  // the beamlets run one after the other from hist_begin
-  HostSource hs;
-  uint64_t total = 0;
+  if (opt.tally_budget < 0) return set_error(DPM_E_INVALID, "negative tally budget");
+  // each beamlet's histories keep the ids they have when the beamlets run one
+  // after the other from hist_begin, whatever group it runs in
+  std::vector<uint64_t> first(nb + 1, 0);
   for (int i = 0; i < nb; ++i) {
     int rc = check_source(t, b[i].particle, b[i].energy, 0, 0);
     if (rc != DPM_OK) return rc;
     if (b[i].nhist == 0) return set_error(DPM_E_INVALID, "beamlet without histories");
-    hs.add(HostEntry{opt.hist_begin + total, b[i].nhist, b[i].x0, b[i].y0, b[i].energy, b[i].particle, i});
-    total += b[i].nhist;
+    first[i + 1] = first[i] + b[i].nhist;
   }
-  if (opt.hist_begin + total > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  if (opt.hist_begin + first[nb] > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  // dense FP64 rows of a group of beamlets within the budget
+  const size_t row_bytes = sizeof(double) * (size_t)p->nvox;
+  const size_t budget = opt.tally_budget > 0 ? (size_t)opt.tally_budget : kBeamletBudget;
+  if (row_bytes > budget)
+    return set_error(DPM_E_INVALID, "one beamlet's dense row (" + std::to_string(row_bytes >> 20) +
+                                        " MiB) exceeds the tally budget");
+  const int group = (int)std::min<size_t>((size_t)nb, budget / row_bytes);
   HIP_TRY(hipSetDevice(t->device));
   const hipStream_t s = t->stream;
   if (t->csr_col) (void)hipFree(t->csr_col);
@@ -1650,73 +1660,150 @@ int dpm_simulate_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   t->csr_col = nullptr;
   t->csr_val = nullptr;
   t->csr_nnz = 0;
-  // dense per-beamlet rows (nb x nvox FP64), cached like the other tallies
   t->last_tally = t->last_second = nullptr;
   t->last_n = 0;
-  const size_t rows_bytes = sizeof(double) * (size_t)nb * p->nvox;
-  const size_t nseg = (size_t)nb * kCsrSeg;
+  const size_t nseg = (size_t)group * kCsrSeg;
   t->rows_zero = 0;   // the beamlet rows are cleared by this run, not left zero
-  if (!t->buf_rows.ensure(rows_bytes) || !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * nseg + 1024))
-    return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string(rows_bytes >> 20) + " MiB)");
+  if (!t->buf_rows.ensure(row_bytes * group) ||
+      !t->buf_blaux.ensure(sizeof(double) * nb + 2 * sizeof(int64_t) * nseg + 1024))
+    return set_error(DPM_E_NOMEM, "cannot allocate the beamlet tally (" + std::to_string((row_bytes * group) >> 20) +
+                                      " MiB)");
   double* d_rows = t->buf_rows.d();
   unsigned char* q = (unsigned char*)t->buf_blaux.p;
   unsigned long long* d_rmax = (unsigned long long*)q; q += ((sizeof(double) * nb + 255) & ~(size_t)255);
   int64_t* d_segcnt = (int64_t*)q; q += ((sizeof(int64_t) * nseg + 255) & ~(size_t)255);
   int64_t* d_segoff = (int64_t*)q;
-  (void)hipMemsetAsync(d_rows, 0, rows_bytes, s);
   (void)hipMemsetAsync(d_rmax, 0, sizeof(double) * nb, s);
   (void)hipMemsetAsync(t->stats, 0, sizeof(StatsDev), s);
-  dpm_stats local;
-  std::memset(&local, 0, sizeof(local));
+  dpm_stats total;
+  std::memset(&total, 0, sizeof(total));
   opt.nbatch = 0;
-  const ChunkPlan plan = plan_chunks(total, 0, opt.chunk, pipe_count(&opt));
-  int rc = opt.precision == DPM_FP64
-               ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, plan, d_rows, nullptr, nullptr, nullptr, nullptr, &local)
-               : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, plan, d_rows, nullptr, nullptr, nullptr, nullptr, &local);
-  if (rc == DPM_OK) {
-    hipLaunchKernelGGL(k_bl_stats, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax, &t->stats->scored);
-    if (hipGetLastError() != hipSuccess) rc = set_error(DPM_E_DEVICE, "beamlet row statistics launch failed");
-  }
-  if (rc == DPM_OK) rc = finish_stats(t, &local);
-  if (rc == DPM_OK) {
-    hipLaunchKernelGGL(k_bl_count, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax, rel_threshold,
-                       d_segcnt);
-    std::vector<int64_t> seg(nseg);
-    // the run stream is non-blocking: copies are ordered on it
-    if (hipMemcpyAsync(seg.data(), d_segcnt, sizeof(int64_t) * nseg, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-      rc = set_error(DPM_E_DEVICE, "CSR count failed");
-    if (rc == DPM_OK) {
-      // segment counts -> global segment offsets and row pointers
-      int64_t acc = 0;
-      row_ptr[0] = 0;
-      for (int i = 0; i < nb; ++i) {
-        for (int k = 0; k < kCsrSeg; ++k) {
-          const int64_t c = seg[(size_t)i * kCsrSeg + k];
-          seg[(size_t)i * kCsrSeg + k] = acc;
-          acc += c;
-        }
-        row_ptr[i + 1] = acc;
-      }
-      const int64_t nnz = acc;
-      if (nnz > 0) {
-        if (hipMalloc(&t->csr_col, sizeof(int32_t) * nnz) != hipSuccess ||
-            hipMalloc(&t->csr_val, sizeof(double) * nnz) != hipSuccess) {
-          rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
-        } else if (hipMemcpyAsync(d_segoff, seg.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s) !=
-                   hipSuccess) {
-          rc = set_error(DPM_E_DEVICE, "CSR offset copy failed");
-        } else {
-          hipLaunchKernelGGL(k_bl_fill, dim3((unsigned)nseg), dim3(256), 0, s, d_rows, p->nvox, d_rmax,
-                             rel_threshold, d_segoff, t->csr_col, t->csr_val);
-          // the staging vector must outlive the copy above
-          if (hipStreamSynchronize(s) != hipSuccess) rc = set_error(DPM_E_DEVICE, "CSR build failed");
-          else t->csr_nnz = nnz;
-        }
+  // the CSR of each group, joined at the end
+  struct Part {
+    int32_t* col = nullptr;
+    double* val = nullptr;
+    int64_t nnz = 0;
+  };
+  std::vector<Part> parts;
+  struct PartsGuard {
+    std::vector<Part>& v;
+    ~PartsGuard() {
+      for (Part& x : v) {
+        if (x.col) (void)hipFree(x.col);
+        if (x.val) (void)hipFree(x.val);
       }
     }
+  } pguard{parts};
+  int rc = DPM_OK;
+  int64_t acc = 0;
+  row_ptr[0] = 0;
+  std::vector<int64_t> seg(nseg);
+  for (int b0 = 0; b0 < nb && rc == DPM_OK; b0 += group) {
+    const int b1 = std::min(nb, b0 + group), ng = b1 - b0;
+    HostSource hs;
+    for (int i = b0; i < b1; ++i)
+      hs.add(HostEntry{opt.hist_begin + first[i], b[i].nhist, b[i].x0, b[i].y0, b[i].energy, b[i].particle, i - b0});
+    const uint64_t nh = first[b1] - first[b0];
+    (void)hipMemsetAsync(d_rows, 0, row_bytes * ng, s);
+    const ChunkPlan plan = plan_chunks(nh, 0, opt.chunk, pipe_count(&opt));
+    dpm_stats local;
+    std::memset(&local, 0, sizeof(local));
+    rc = opt.precision == DPM_FP64
+             ? run_transport<double>(t, p, hs, &opt, t->wsd, t->td, plan, d_rows, nullptr, nullptr, nullptr, nullptr,
+                                     &local)
+             : run_transport<float>(t, p, hs, &opt, t->wsf, t->tf, plan, d_rows, nullptr, nullptr, nullptr, nullptr,
+                                    &local);
+    if (rc != DPM_OK) break;
+    total.seconds += local.seconds;
+    total.rounds += local.rounds;
+    total.graphs += local.graphs;
+    total.lds_tables = local.lds_tables;
+    for (int k = 0; k < 4; ++k) {
+      total.kernel_ms[k] += local.kernel_ms[k];
+      total.kernel_calls[k] += local.kernel_calls[k];
+    }
+    const unsigned ngs = (unsigned)((size_t)ng * kCsrSeg);
+    hipLaunchKernelGGL(k_bl_stats, dim3(ngs), dim3(256), 0, s, d_rows, p->nvox, d_rmax + b0, &t->stats->scored);
+    hipLaunchKernelGGL(k_bl_count, dim3(ngs), dim3(256), 0, s, d_rows, p->nvox, d_rmax + b0, rel_threshold,
+                       d_segcnt);
+    if (hipGetLastError() != hipSuccess) { rc = set_error(DPM_E_DEVICE, "beamlet CSR count launch failed"); break; }
+    // the run stream is non-blocking: copies are ordered on it
+    if (hipMemcpyAsync(seg.data(), d_segcnt, sizeof(int64_t) * ngs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = set_error(DPM_E_DEVICE, "CSR count failed");
+      break;
+    }
+    // segment counts -> offsets within the group's arrays, and row pointers
+    int64_t g = 0;
+    for (int i = 0; i < ng; ++i) {
+      for (int k = 0; k < kCsrSeg; ++k) {
+        const int64_t c = seg[(size_t)i * kCsrSeg + k];
+        seg[(size_t)i * kCsrSeg + k] = g;
+        g += c;
+      }
+      row_ptr[b0 + i + 1] = acc + g;
+    }
+    acc += g;
+    Part part;
+    part.nnz = g;
+    if (g > 0) {
+      if (hipMalloc(&part.col, sizeof(int32_t) * g) != hipSuccess || hipMalloc(&part.val, sizeof(double) * g) != hipSuccess) {
+        parts.push_back(part);
+        rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
+        break;
+      }
+      if (hipMemcpyAsync(d_segoff, seg.data(), sizeof(int64_t) * ngs, hipMemcpyHostToDevice, s) != hipSuccess) {
+        parts.push_back(part);
+        rc = set_error(DPM_E_DEVICE, "CSR offset copy failed");
+        break;
+      }
+      hipLaunchKernelGGL(k_bl_fill, dim3(ngs), dim3(256), 0, s, d_rows, p->nvox, d_rmax + b0, rel_threshold, d_segoff,
+                         part.col, part.val);
+      // the staging vector is reused by the next group: complete the copy first
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        parts.push_back(part);
+        rc = set_error(DPM_E_DEVICE, "CSR build failed");
+        break;
+      }
+    }
+    parts.push_back(part);
   }
-  if (st) *st = local;
+  if (rc == DPM_OK) rc = finish_stats(t, &total);
+  if (rc == DPM_OK && acc > 0) {
+    if (parts.size() == 1) {
+      // one group: its arrays are the result
+      t->csr_col = parts[0].col;
+      t->csr_val = parts[0].val;
+      parts[0].col = nullptr;
+      parts[0].val = nullptr;
+      t->csr_nnz = acc;
+    } else if (hipMalloc(&t->csr_col, sizeof(int32_t) * acc) != hipSuccess ||
+               hipMalloc(&t->csr_val, sizeof(double) * acc) != hipSuccess) {
+      rc = set_error(DPM_E_NOMEM, "cannot allocate the CSR output");
+    } else {
+      int64_t o = 0;
+      for (const Part& x : parts) {
+        if (x.nnz == 0) continue;
+        if (hipMemcpyAsync(t->csr_col + o, x.col, sizeof(int32_t) * x.nnz, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(t->csr_val + o, x.val, sizeof(double) * x.nnz, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+          rc = set_error(DPM_E_DEVICE, "CSR join failed");
+          break;
+        }
+        o += x.nnz;
+      }
+      if (rc == DPM_OK && hipStreamSynchronize(s) != hipSuccess) rc = set_error(DPM_E_DEVICE, "CSR join failed");
+      if (rc == DPM_OK) t->csr_nnz = acc;
+    }
+  }
+  if (rc != DPM_OK) {
+    if (t->csr_col) (void)hipFree(t->csr_col);
+    if (t->csr_val) (void)hipFree(t->csr_val);
+    t->csr_col = nullptr;
+    t->csr_val = nullptr;
+    t->csr_nnz = 0;
+  }
+  total.beamlet_groups = (int)((nb + group - 1) / group);
+  if (st) *st = total;
   return rc;
 }
 
@@ -1858,7 +1945,9 @@ int dpm_autotune_beamlets(dpm_tables* t, const dpm_phantom* p, const dpm_beamlet
   if (nb > (1 << 20)) return set_error(DPM_E_INVALID, "too many beamlets");
   if (nsample < 1000 || nsample > ((uint64_t)1 << 24)) return set_error(DPM_E_INVALID, "sample of 1e3 .. 2^24 histories");
   if (precision != DPM_FP32 && precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
-  // every beamlet, its history count scaled so that the total is ~nsample
+  // the beamlets of one group of a real run (dense rows within the default
+  // budget), each with its history count scaled so that the total is ~nsample
+  nb = (int)std::min<size_t>((size_t)nb, std::max<size_t>(1, kBeamletBudget / (sizeof(double) * (size_t)p->nvox)));
   uint64_t total = 0;
   for (int i = 0; i < nb; ++i) {
     int rc = check_source(t, b[i].particle, b[i].energy, 0, 0);

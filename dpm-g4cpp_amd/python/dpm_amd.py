@@ -52,7 +52,7 @@ class RunOptions(ctypes.Structure):
                 ("nbatch", ctypes.c_int), ("chunk", ctypes.c_int64), ("waves_per_cu", ctypes.c_int),
                 ("hist_edep", ctypes.c_void_p), ("refill_min", ctypes.c_int), ("tally_mode", ctypes.c_int),
                 ("graph", ctypes.c_int), ("rounds_per_sync", ctypes.c_int), ("table_memory", ctypes.c_int),
-                ("pipes", ctypes.c_int)]
+                ("pipes", ctypes.c_int), ("tally_budget", ctypes.c_int64)]
 
 
 class Stats(ctypes.Structure):
@@ -61,7 +61,8 @@ class Stats(ctypes.Structure):
                 ("rounds", ctypes.c_int), ("hist_size", ctypes.c_int), ("kernel_ms", ctypes.c_double * 4),
                 ("kernel_calls", ctypes.c_int * 4), ("counters_photon", ctypes.c_uint64 * 10),
                 ("counters_electron", ctypes.c_uint64 * 10), ("graphs", ctypes.c_int), ("lds_tables", ctypes.c_int),
-                ("lane_slots", ctypes.c_uint64 * 2), ("guard_exits", ctypes.c_uint64)]
+                ("lane_slots", ctypes.c_uint64 * 2), ("beamlet_groups", ctypes.c_int),
+                ("guard_exits", ctypes.c_uint64)]
 
     def as_dict(self):
         return dict(seconds=self.seconds, e_scored=self.e_scored, e_escaped=self.e_escaped,
@@ -71,7 +72,8 @@ class Stats(ctypes.Structure):
                     counters_electron=dict(zip(COUNTER_NAMES, list(self.counters_electron))),
                     rounds=self.rounds, hist_size=self.hist_size, kernel_ms=list(self.kernel_ms),
                     kernel_calls=list(self.kernel_calls), graphs=self.graphs, lds_tables=self.lds_tables,
-                    lane_slots=list(self.lane_slots), guard_exits=self.guard_exits)
+                    lane_slots=list(self.lane_slots), beamlet_groups=self.beamlet_groups,
+                    guard_exits=self.guard_exits)
 
 
 class Schedule(ctypes.Structure):
@@ -410,8 +412,9 @@ def autotune_beamlets(tables, phantom, beamlets, nsample=4000000, precision=DPM_
 
 def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, precision=DPM_FP32, chunk=0,
                       waves_per_cu=0, refill_min=0, tally_mode=0, schedule=None, hist_begin=0, graph=True,
-                      table_memory=0, pipes=0):
-    """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats)."""
+                      table_memory=0, pipes=0, tally_budget=0):
+    """beamlets: list of (x0, y0, particle, energy, nhist). Returns (row_ptr, col, val, stats).
+    tally_budget: device bytes for the dense rows of the beamlets run together (0: 16 GiB)."""
     arr = (Beamlet * len(beamlets))(*[Beamlet(x0=b[0], y0=b[1], particle=b[2], energy=b[3], nhist=b[4])
                                        for b in beamlets])
     if schedule is not None:
@@ -420,7 +423,7 @@ def simulate_beamlets(tables, phantom, beamlets, rel_threshold=1e-3, seed=1, pre
         pipes = schedule.get("pipes", 0)
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, chunk=chunk, waves_per_cu=waves_per_cu,
                      refill_min=refill_min, tally_mode=tally_mode, graph=int(bool(graph)),
-                     table_memory=table_memory, pipes=pipes)
+                     table_memory=table_memory, pipes=pipes, tally_budget=int(tally_budget))
     row_ptr = np.zeros(len(beamlets) + 1, dtype=np.int64)
     st = Stats()
     _check(lib().dpm_simulate_beamlets(tables.h, phantom.h, arr, len(beamlets), ctypes.byref(opt), rel_threshold,

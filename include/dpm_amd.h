@@ -199,6 +199,10 @@ typedef struct dpm_run_options {
                                       their own particle queues and batch rows -- side by side, so
                                       that one pipe's kernel tails, drain rounds and folds overlap the
                                       other's transport; the same results either way */
+  int64_t tally_budget;            /* dpm_simulate_beamlets: device bytes for the dense FP64 rows of
+                                      the beamlets run together (0 = 16 GiB); beamlets beyond
+                                      budget / (8 * voxels) run in further groups, with the same
+                                      history ids and the same CSR */
 } dpm_run_options;
 
 enum { DPM_TABLES_AUTO = 0, DPM_TABLES_GLOBAL = 1 };
@@ -227,6 +231,7 @@ typedef struct dpm_stats {
   uint64_t lane_slots[2];          /* photon / electron kernels: 64 per transport-loop trip of a
                                       wave; the steps (counters_*[DPM_CNT_G_STEPS / _E_STEPS]) over
                                       this are the lane utilisation */
+  int beamlet_groups;              /* dpm_simulate_beamlets: groups the beamlets ran in */
   uint64_t guard_exits;            /* particles ended by a termination guard instead of the physics:
                                       the per-particle cap of 2e5 transport steps, or a boundary /
                                       locate loop out of tolerance pushes (16); their energy is counted
@@ -290,7 +295,9 @@ int dpm_simulate_noalloc(dpm_tables* t, const dpm_phantom* p, const dpm_source* 
 int dpm_tally_fetch(dpm_tables* t, double* tally, double* second);
 
 /* Batched beamlets: `nb` pencil beamlets (entry point, particle, energy,
- * history count) in one run, scored into a sparse dose-influence matrix:
+ * history count) in one run -- or in groups of as many as the dense rows of
+ * opt->tally_budget hold, each group one run -- scored into a sparse
+ * dose-influence matrix:
  * for each beamlet the voxels above `rel_threshold` of its maximum, as CSR
  * (row_ptr[nb+1]; nnz = row_ptr[nb]). The column indices and values stay on
  * the device in the tables handle until dpm_beamlets_fetch copies them into

@@ -123,6 +123,13 @@ def test_config4_full_size_beamlets(gpu_tables0):
     # the thresholded rows hold part of the scored energy, never more
     assert csr_total <= st["e_scored"] * (1 + 1e-9)
     assert csr_total > 0.5 * st["e_scored"]
+    # the same beamlets in 4 groups of 128 dense rows: the same CSR
+    g4 = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, rel_threshold=thr, seed=8, tally_budget=128 * 8 * ph.nbins)
+    assert g4[3]["beamlet_groups"] == 4
+    np.testing.assert_array_equal(g4[0], row_ptr)
+    np.testing.assert_array_equal(g4[1], col)
+    np.testing.assert_allclose(g4[2], val, rtol=1e-12)
+    del g4
     # four rows (the largest, the smallest, an electron and a photon beamlet)
     # against plain per-beamlet runs of the same history ids
     off = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
@@ -141,4 +148,41 @@ def test_config4_full_size_beamlets(gpu_tables0):
         np.testing.assert_allclose(v[o], d[keep], rtol=1e-9)
         print("config 4: beamlet %d (%s %.1f MeV, %d histories) equals its plain run: %d voxels"
               % (b, "e-" if p == -1 else "gamma", e, n, keep.size), flush=True)
+    ph.close()
+
+
+@pytest.mark.gpu
+def test_config4_beamlets_on_the_config5_phantom(gpu_tables3):
+    # the 512 beamlets of configuration 4 on the 512x512x256 phantom at 1 mm
+    # (67 M voxels, 537 MB per dense FP64 row): the default 16 GiB budget runs
+    # them in groups of 31 rows; energy balance, CSR structure and two rows
+    # against plain runs
+    mat, rho = phantoms.heterogeneous(512, 512, 256, 1.0, seed=2024, axis_scale=2.0)
+    ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 1.0)
+    del mat
+    bl = phantoms.beamlets(512, seed=4321)
+    thr = 1e-3
+    print("config 4 beamlets on the config-5 phantom: %d histories" % sum(b[4] for b in bl), flush=True)
+    row_ptr, col, val, st = dpm_amd.simulate_beamlets(gpu_tables3, ph, bl, rel_threshold=thr, seed=8)
+    per_group = (16 << 30) // (8 * ph.nbins)
+    assert st["beamlet_groups"] == -(-512 // per_group), st["beamlet_groups"]
+    assert st["guard_exits"] == 0
+    bal = _balance(st, st["e_primary"])
+    assert abs(bal) < 1e-5 * st["e_primary"], bal
+    assert row_ptr[0] == 0 and np.all(np.diff(row_ptr) >= 1) and row_ptr[-1] == col.size
+    assert col.min() >= 0 and col.max() < ph.nbins and np.all(val > 0)
+    assert val.sum() <= st["e_scored"] * (1 + 1e-9)
+    off = np.concatenate([[0], np.cumsum([b[4] for b in bl])])
+    for b in (0, per_group + 1):   # in the first and in the second group
+        x, y, p, e, n = bl[b]
+        d, _, _, _ = dpm_amd.simulate(gpu_tables3, ph, n, particle=p, energy=e, x0=x, y0=y, seed=8,
+                                      hist_begin=int(off[b]))
+        d = d.ravel()
+        keep = np.nonzero((d > thr * d.max()) & (d > 0))[0]
+        c, v = col[row_ptr[b]:row_ptr[b + 1]], val[row_ptr[b]:row_ptr[b + 1]]
+        o = np.argsort(c)
+        np.testing.assert_array_equal(c[o], keep)
+        np.testing.assert_allclose(v[o], d[keep], rtol=1e-9)
+    print("config 4 on the config-5 phantom: %d groups, %d nonzeros, %.2f s device" %
+          (st["beamlet_groups"], col.size, st["seconds"]), flush=True)
     ph.close()

@@ -174,6 +174,29 @@ def test_beamlets_equal_plain_runs(gpu_tables0):
 
 
 @pytest.mark.gpu
+def test_beamlet_groups_give_the_single_group_csr(gpu_tables0):
+    # a tally budget of 3 dense rows runs 7 beamlets in groups of 3, 3 and 1:
+    # the same history ids, so the same CSR as one group (to the order of FP64
+    # atomic additions) and the same event counts
+    mat, rho = water_cube(24)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, 2.0)
+    bl = [(-6.0, 4.0, -1, 9.0, 3000), (10.0, -2.0, 0, 12.0, 5000), (0.0, 0.0, -1, 15.0, 2000),
+          (4.0, 4.0, 0, 2.0, 4000), (-8.0, -8.0, -1, 6.0, 2500), (2.0, -9.0, 0, 18.0, 1500), (7.0, 7.0, -1, 3.0, 3500)]
+    one = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, rel_threshold=1e-3, seed=12)
+    grp = dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, rel_threshold=1e-3, seed=12, tally_budget=3 * 8 * ph.nbins)
+    assert one[3]["beamlet_groups"] == 1 and grp[3]["beamlet_groups"] == 3
+    np.testing.assert_array_equal(grp[0], one[0])
+    np.testing.assert_array_equal(grp[1], one[1])
+    np.testing.assert_allclose(grp[2], one[2], rtol=1e-12)
+    assert grp[3]["counters"] == one[3]["counters"]
+    np.testing.assert_allclose(grp[3]["e_scored"], one[3]["e_scored"], rtol=1e-12)
+    np.testing.assert_allclose(grp[3]["e_primary"], one[3]["e_primary"], rtol=1e-12)
+    with pytest.raises(dpm_amd.DpmError):
+        dpm_amd.simulate_beamlets(gpu_tables0, ph, bl, tally_budget=8 * ph.nbins - 8)
+    ph.close()
+
+
+@pytest.mark.gpu
 def test_schedules_do_not_change_results(gpu_tables0):
     # per-particle streams: chunk, waves per CU, refill threshold and tally
     # mode change only the order of FP64 tally additions
