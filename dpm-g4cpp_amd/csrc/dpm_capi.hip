@@ -668,12 +668,14 @@ struct HostEntry {
   int particle;
   int row;          // tally row, or -1 (batch row / row 0)
   double angle_deg = 0.0;   // gantry fields
+  double z = 0.0, u = 0.0, v = 0.0, w = 1.0;   // phase-space records: entry z and direction
 };
 struct HostSource {
   int kind = DPM_SRC_PENCIL;
   double emin = 0, emax = 0, fx = 0, fy = 0;
   int spectrum = DPM_SPEC_FLAT;
   double emode = 0;
+  int phsp_project = 0;             // phase-space records outside a voxel phantom move to its surface
   std::vector<double> spec_edges, spec_cdf;   // DPM_SPEC_TABLE
   std::vector<HostEntry> list[2];   // [0] photons, [1] e-/e+
   void add(const HostEntry& h) { list[h.particle == DPM_PHOTON ? 0 : 1].push_back(h); }
@@ -717,11 +719,13 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   bytes += (8 * nsp + 255) & ~(size_t)255;
   const size_t off_cdf = bytes;
   bytes += (8 * nsp + 255) & ~(size_t)255;
-  size_t off[2][9];
+  constexpr int kArrays = 13;
+  size_t off[2][kArrays];
   for (int k = 0; k < 2; ++k) {
     const size_t n = hs.list[k].size();
-    const size_t sz[9] = {8 * (n + 1), 8 * n, 8 * n, 8 * n, 8 * n, 4 * n, 4 * n, 8 * n, 8 * n};
-    for (int a = 0; a < 9; ++a) {
+    const size_t sz[kArrays] = {8 * (n + 1), 8 * n, 8 * n, 8 * n, 8 * n, 4 * n, 4 * n, 8 * n, 8 * n,
+                                8 * n, 8 * n, 8 * n, 8 * n};
+    for (int a = 0; a < kArrays; ++a) {
       off[k][a] = bytes;
       bytes += (sz[a] + 255) & ~(size_t)255;
     }
@@ -739,6 +743,7 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   SourceParams S;
   std::memset(&S, 0, sizeof(S));
   S.kind = hs.kind;
+  S.phsp_project = hs.phsp_project;
   S.nbatch = nbatch;
   S.emin = hs.emin;
   S.emax = hs.emax;
@@ -779,6 +784,10 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     int* row = (int*)(h.data() + off[k][6]);
     double* ca = (double*)(h.data() + off[k][7]);
     double* sa = (double*)(h.data() + off[k][8]);
+    double* pz = (double*)(h.data() + off[k][9]);
+    double* du = (double*)(h.data() + off[k][10]);
+    double* dv = (double*)(h.data() + off[k][11]);
+    double* dw = (double*)(h.data() + off[k][12]);
     cum[0] = 0;
     for (int i = 0; i < n; ++i) {
       cum[i + 1] = cum[i] + L[i].n;
@@ -787,6 +796,7 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
       pt[i] = L[i].particle;
       row[i] = L[i].row;
       gantry_cos_sin(L[i].angle_deg, &ca[i], &sa[i]);
+      pz[i] = L[i].z; du[i] = L[i].u; dv[i] = L[i].v; dw[i] = L[i].w;
     }
     PrimList& D = S.list[k];
     D.n = n;
@@ -799,6 +809,10 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
     D.row = (const int*)(base + off[k][6]);
     D.ca = (const double*)(base + off[k][7]);
     D.sa = (const double*)(base + off[k][8]);
+    D.z = (const double*)(base + off[k][9]);
+    D.u = (const double*)(base + off[k][10]);
+    D.v = (const double*)(base + off[k][11]);
+    D.w = (const double*)(base + off[k][12]);
   }
   std::memcpy(h.data(), &S, sizeof(S));
   // ordered on the run stream, completed before the staging vector goes out
@@ -855,13 +869,19 @@ int run_transport(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, con
 
   const uint64_t n_ph = hs.count(0), n_el = hs.count(1);
   const uint64_t nhist = n_ph + n_el;
-  // batch rows need one source entry (history index = history id - begin)
-  // batch rows follow history ids: one particle kind (one list, entries in id
-  // order) and no per-entry rows
-  bool single = hs.list[0].empty() != hs.list[1].empty();
-  for (int k = 0; k < 2; ++k)
-    for (const HostEntry& h : hs.list[k]) single = single && h.row < 0;
-  const int nbatch = (single && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
+  // batch rows follow history ids, so the ids must run through the entries
+  // in the order the chunks take them ([photon list | e-/e+ list], each in
+  // list order, consecutive from hist_begin), and no entry has its own row
+  bool ordered = true;
+  {
+    uint64_t next = opt->hist_begin;
+    for (int k = 0; k < 2; ++k)
+      for (const HostEntry& h : hs.list[k]) {
+        ordered = ordered && h.row < 0 && h.first == next;
+        next += h.n;
+      }
+  }
+  const int nbatch = (ordered && p->geom == GEOM_VOXEL && opt->nbatch > 1) ? opt->nbatch : 0;
   const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(plan.chunk, kDefaultChunk));
   const int nrows_per_chunk = std::max(1, plan.rows);
   const int npipes = pipe_count(opt);
@@ -1463,6 +1483,48 @@ int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* s
   if (opt.hist_begin + total > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
   if (opt.nbatch > 1 && (uint64_t)opt.nbatch > total) return set_error(DPM_E_INVALID, "more batches than histories");
   return simulate_source(t, p, hs, total, opt, tally, steps, st);
+}
+
+int dpm_simulate_phsp(dpm_tables* t, const dpm_phantom* p, const dpm_phsp_particle* rec, uint64_t n,
+                      const dpm_run_options* opt_in, double* tally, double* steps, dpm_stats* st) {
+  if (!t || !p || !rec) return set_error(DPM_E_INVALID, "null argument");
+  if (p->owner != t) return set_error(DPM_E_INVALID, "phantom belongs to another tables handle");
+  if (n == 0) return set_error(DPM_E_INVALID, "no phase-space records");
+  if (n > ((uint64_t)1 << 30)) return set_error(DPM_E_INVALID, "at most 2^30 phase-space records per call");
+  dpm_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  if (opt_in) opt = *opt_in;
+  if (opt.hist_begin + n > 0x100000000ull) return set_error(DPM_E_INVALID, "history ids exceed 32 bits");
+  if (opt.precision != DPM_FP32 && opt.precision != DPM_FP64) return set_error(DPM_E_INVALID, "unknown precision");
+  if (opt.pipes < 0 || opt.pipes > kPipes) return set_error(DPM_E_INVALID, "pipes must be 0, 1 or 2");
+  if (opt.nbatch > 1 && (uint64_t)opt.nbatch > n) return set_error(DPM_E_INVALID, "more batches than histories");
+  HostSource hs;
+  hs.kind = 3;
+  hs.phsp_project = p->geom == GEOM_VOXEL;
+  uint64_t nph = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const dpm_phsp_particle& r = rec[i];
+    int rc = check_source(t, r.particle, r.energy, 0, 0);
+    if (rc != DPM_OK) return set_error(rc, "phase-space record " + std::to_string(i) + ": " + g_last_error);
+    const double nn = r.u * r.u + r.v * r.v + r.w * r.w;
+    if (!std::isfinite(r.x) || !std::isfinite(r.y) || !std::isfinite(r.z) || !(std::fabs(nn - 1.0) < 1e-6))
+      return set_error(DPM_E_INVALID, "phase-space record " + std::to_string(i) +
+                                          ": position not finite or direction not a unit vector");
+    nph += r.particle == DPM_PHOTON;
+  }
+  // history ids: the photon records first, then the e-/e+ records, each in
+  // the caller's order (the order the chunks run them)
+  hs.list[0].reserve(nph);
+  hs.list[1].reserve(n - nph);
+  uint64_t iph = 0, iel = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const dpm_phsp_particle& r = rec[i];
+    const uint64_t id = opt.hist_begin + (r.particle == DPM_PHOTON ? iph++ : nph + iel++);
+    HostEntry h{id, 1, r.x, r.y, r.energy, r.particle, -1};
+    h.z = r.z; h.u = r.u; h.v = r.v; h.w = r.w;
+    hs.add(h);
+  }
+  return simulate_source(t, p, hs, n, opt, tally, steps, st);
 }
 
 static int simulate_source(dpm_tables* t, const dpm_phantom* p, const HostSource& hs, uint64_t nhist,

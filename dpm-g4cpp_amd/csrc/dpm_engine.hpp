@@ -106,11 +106,19 @@ struct PrimList {
   const int* row;           // [n] tally row, or -1: batch row (or row 0)
   const double* ca;         // [n] gantry fields: cos and sin of the angle
   const double* sa;
+  const double* z;          // [n] phase-space records: entry z (mm) and direction
+  const double* u;
+  const double* v;
+  const double* w;
 };
 
 // Source of one run, resident in device memory for its duration
 struct SourceParams {
-  int kind;                 // 0 pencil, 1 rectangular field around the entry point, 2 gantry field
+  int kind;                 // 0 pencil, 1 rectangular field around the entry point, 2 gantry field,
+                            // 3 phase-space records (one history per list entry, its own position,
+                            // direction, energy and particle)
+  int phsp_project;         // 3, voxel phantom: records outside the phantom move along their direction
+                            // to where they enter it
   int nbatch;               // > 1: tally row = batch index - first batch of the chunk
   double emin, emax;        // emax > emin: energy from the spectrum over [emin, emax)
   int spectrum;             // 0 flat, 1 triangular with mode emode, 2 table

@@ -510,8 +510,33 @@ __device__ __forceinline__ double spectrum_table(int n, const double* __restrict
   return edges[lo] + f * (edges[lo + 1] - edges[lo]);
 }
 
-// returns false when a gantry ray misses the phantom (t.E then holds the
-// primary's energy, which the caller counts as escaped)
+// A phase-space record outside the voxel phantom moves along its direction
+// to where that ray enters the phantom (slab method), 1e-3 mm inside; false
+// if the ray misses the phantom. A record inside or on a face stays where it
+// is (a record on the source plane starts as the pencil beam does).
+__device__ __forceinline__ bool phsp_entry(const Geometry<R>& G, R pos[3], const R dir[3]) {
+  const R h[3] = {-G.org[0], -G.org[1], -G.org[2]};
+  if (rabs(pos[0]) <= h[0] && rabs(pos[1]) <= h[1] && rabs(pos[2]) <= h[2]) return true;
+  R tin = (R)-1e30, tout = (R)1e30;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (dir[i] == (R)0) {
+      if (pos[i] <= -h[i] || pos[i] >= h[i]) return false;
+      continue;
+    }
+    const R t1 = (-h[i] - pos[i]) / dir[i], t2 = (h[i] - pos[i]) / dir[i];
+    tin = rmax(tin, rmin(t1, t2));
+    tout = rmin(tout, rmax(t1, t2));
+  }
+  if (!(tin >= (R)0 && tin + (R)2e-3 < tout)) return false;
+  const R t = tin + (R)1e-3;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pos[i] = pos[i] + t * dir[i];
+  return true;
+}
+
+// returns false when a gantry ray or a phase-space record misses the phantom
+// (t.E then holds the primary's energy, which the caller counts as escaped)
 __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourceParams* __restrict__ S,
                                             const PrimList& L, uint32_t batch_first, uint64_t v, Track& t,
                                             double& eprim, int k) {
@@ -519,10 +544,14 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   // one pencil beam's axis voxels do not receive a wave's worth of atomics
   if (S->perm[k]) v = (v * S->perm[k]) % S->count[k];
   int lo = 0, hi = L.n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (L.cum[mid] <= v) lo = mid;
-    else hi = mid - 1;
+  if (S->kind == 3) {
+    lo = (int)v;   // one history per record
+  } else {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (L.cum[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
   }
   const uint64_t hid = L.first[lo] + (v - L.cum[lo]);
   t.rng.k0 = P.k0; t.rng.k1 = P.k1;
@@ -534,14 +563,16 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   const double x0 = L.x[lo], y0 = L.y[lo];
   t.pos[0] = (R)x0;
   t.pos[1] = (R)y0;
-  t.pos[2] = (R)S->z0;
-  if (S->kind >= 1) {
+  t.pos[2] = S->kind == 3 ? (R)L.z[lo] : (R)S->z0;
+  if (S->kind == 1 || S->kind == 2) {
     const R ux = t.rng.uni<R>(), uy = t.rng.uni<R>();
     t.pos[0] = (R)x0 + (ux - (R)0.5) * (R)S->fx;
     t.pos[1] = (R)y0 + (uy - (R)0.5) * (R)S->fy;
   }
   R E = (R)L.e[lo];
-  if (S->spectrum == 2)
+  if (S->kind == 3) {
+    // a phase-space record: its own energy and direction, no source draws
+  } else if (S->spectrum == 2)
     E = (R)spectrum_table(S->spec_n, S->spec_edges, S->spec_cdf, t.rng.draw24());
   else if (S->emax > S->emin)
     E = S->spectrum == 1 ? triangular(t.rng.uni<R>(), (R)S->emin, (R)S->emode, (R)S->emax)
@@ -550,6 +581,10 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   t.dir[0] = (R)0; t.dir[1] = (R)0; t.dir[2] = (R)1;
   bool inside = true;
   if (S->kind == 2) inside = gantry_entry(t.pos[0], t.pos[1], (R)L.ca[lo], (R)L.sa[lo], P.G, t.pos, t.dir);
+  if (S->kind == 3) {
+    t.dir[0] = (R)L.u[lo]; t.dir[1] = (R)L.v[lo]; t.dir[2] = (R)L.w[lo];
+    if (S->phsp_project) inside = phsp_entry(P.G, t.pos, t.dir);
+  }
   int row = L.row[lo];
   if (row < 0)
     row = S->nbatch > 1

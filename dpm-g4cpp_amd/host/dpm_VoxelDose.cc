@@ -13,6 +13,7 @@
 //   -S spectrum file (lines "E_low E_high weight" [MeV], contiguous bins)
 //   -n histories  -k batches  -s seed  -f FP64 transport  -g device  -o output
 //   -B beamlet list (lines "x y particle energy histories"; writes CSR)  -t CSR threshold
+//   -P phase-space file (lines "particle E x y z u v w": one history per record)
 #include <getopt.h>
 
 #include <cmath>
@@ -32,7 +33,7 @@
 namespace {
 
 struct Args {
-  std::string data = "./data", mat_file, rho_file, beamlets, out = "dose.bin", particle = "gamma", spectrum;
+  std::string data = "./data", mat_file, rho_file, beamlets, out = "dose.bin", particle = "gamma", spectrum, phsp;
   int nx = 0, ny = 0, nz = 0, nbatch = 10, device = 0;
   double lbox = 2.0, energy = 10.0, emin = 0.0, emax = 0.0, field = 0.0, x0 = 0.0, y0 = 0.0, threshold = 1e-3;
   double angle = 0.0, emode = 0.0;
@@ -63,6 +64,8 @@ void Help() {
                "  -g dev     HIP device                               (default 0)\n"
                "  -B file    beamlet list: lines 'x y particle energy histories' -> CSR output\n"
                "  -t rel     CSR threshold relative to each beamlet's maximum (default 1e-3)\n"
+               "  -P file    phase-space file: lines 'particle E x y z u v w' (MeV, mm, direction\n"
+               "             cosines; '#' comments), one history per record, instead of -p -e -F ...\n"
                "  -o file    output                                   (default dose.bin)\n"
                "  -h         this help\n\n";
 }
@@ -94,6 +97,41 @@ void Check(int rc, const char* what) {
 }
 
 }  // namespace
+
+// phase-space file: one record per line, "particle E x y z u v w" (particle
+// e-, gamma or e+; MeV; mm in phantom coordinates; direction cosines); '#'
+// starts a comment line; any other line that is not exactly these eight
+// fields is an error
+bool ReadPhaseSpace(const std::string& path, std::vector<dpm_phsp_particle>& rec) {
+  std::ifstream f(path);
+  if (!f) {
+    std::cerr << " *** cannot read the phase-space file " << path << std::endl;
+    return false;
+  }
+  std::string line;
+  int lineno = 0;
+  while (std::getline(f, line)) {
+    ++lineno;
+    const size_t first = line.find_first_not_of(" \t\r");
+    if (first == std::string::npos || line[first] == '#') continue;
+    std::istringstream is(line);
+    std::string p, rest;
+    dpm_phsp_particle r{};
+    if (!(is >> p >> r.energy >> r.x >> r.y >> r.z >> r.u >> r.v >> r.w) || (is >> rest) ||
+        (p != "e-" && p != "gamma" && p != "e+")) {
+      std::cerr << " *** " << path << ":" << lineno << ": expected 'particle E x y z u v w', got '" << line << "'"
+                << std::endl;
+      return false;
+    }
+    r.particle = p == "e-" ? DPM_ELECTRON : (p == "gamma" ? DPM_PHOTON : DPM_POSITRON);
+    rec.push_back(r);
+  }
+  if (rec.empty()) {
+    std::cerr << " *** no records in the phase-space file " << path << std::endl;
+    return false;
+  }
+  return true;
+}
 
 // spectrum file: one bin per line, "E_low E_high weight" in MeV, each bin
 // starting where the previous ends; '#' starts a comment line
@@ -137,7 +175,7 @@ static bool ReadSpectrum(const std::string& path, std::vector<double>& edges, st
 int main(int argc, char* argv[]) {
   Args a;
   int c;
-  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:S:n:k:s:fg:B:t:o:")) != -1) {
+  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:S:n:k:s:fg:B:t:o:P:")) != -1) {
     switch (c) {
       case 'd': a.data = optarg; break;
       case 'x': a.nx = std::atoi(optarg); break;
@@ -161,6 +199,7 @@ int main(int argc, char* argv[]) {
       case 'A': a.angle = std::atof(optarg); a.gantry = true; break;
       case 'T': a.emode = std::atof(optarg); a.triangular = true; break;
       case 'S': a.spectrum = optarg; break;
+      case 'P': a.phsp = optarg; break;
       case 'n': a.nhist = (uint64_t)std::atof(optarg); break;
       case 'k': a.nbatch = std::atoi(optarg); break;
       case 's': a.seed = (uint64_t)std::atoll(optarg); break;
@@ -184,6 +223,8 @@ int main(int argc, char* argv[]) {
   // input files are checked before the device is touched
   std::vector<double> sp_edges, sp_weights;
   if (!a.spectrum.empty() && !ReadSpectrum(a.spectrum, sp_edges, sp_weights)) return EXIT_FAILURE;
+  std::vector<dpm_phsp_particle> phsp;
+  if (!a.phsp.empty() && !ReadPhaseSpace(a.phsp, phsp)) return EXIT_FAILURE;
   Check(dpm_tables_load(a.data.c_str(), a.device, &t), "loading tables");
   std::vector<uint8_t> mat = a.mat_file.empty() ? std::vector<uint8_t>(nvox, 0) : ReadRaw<uint8_t>(a.mat_file, nvox);
   std::vector<float> rho;
@@ -269,7 +310,12 @@ int main(int argc, char* argv[]) {
     opt.nbatch = a.nbatch;
     std::vector<double> edep(nvox), sum2(nvox);
     dpm_stats st;
-    Check(dpm_simulate(t, ph, &src, a.nhist, &opt, edep.data(), sum2.data(), &st), "run");
+    if (!phsp.empty()) {
+      a.nhist = phsp.size();   // one history per record
+      Check(dpm_simulate_phsp(t, ph, phsp.data(), phsp.size(), &opt, edep.data(), sum2.data(), &st), "run");
+    } else {
+      Check(dpm_simulate(t, ph, &src, a.nhist, &opt, edep.data(), sum2.data(), &st), "run");
+    }
     std::vector<double> dose, rel;
     VoxelDoseAndSigma(edep.data(), a.nbatch > 1 ? sum2.data() : nullptr, a.nbatch, rho.data(), (int64_t)nvox, a.lbox,
                       a.nhist, dose, rel);

@@ -27,7 +27,7 @@ EXPORTED = ("dpm_tables_create", "dpm_tables_load", "dpm_tables_destroy", "dpm_t
             "dpm_simulate", "dpm_simulate_beamlets", "dpm_beamlets_fetch", "dpm_sample", "dpm_eval", "dpm_host_alloc", "dpm_host_free", "dpm_autotune",
             "dpm_autotune_beamlets", "dpm_tally_fetch", "dpm_perform", "dpm_simulate_fields", "dpm_last_error",
             "dpm_abi_version", "dpm_probe_atomics", "dpm_phantom_set_majorant", "dpm_phantom_get_majorant",
-            "dpm_simulate_noalloc")
+            "dpm_simulate_noalloc", "dpm_simulate_phsp")
 
 
 class DpmError(RuntimeError):
@@ -85,6 +85,12 @@ class Schedule(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PhspParticle(ctypes.Structure):
+    _fields_ = [("particle", ctypes.c_int), ("energy", ctypes.c_double), ("x", ctypes.c_double),
+                ("y", ctypes.c_double), ("z", ctypes.c_double), ("u", ctypes.c_double), ("v", ctypes.c_double),
+                ("w", ctypes.c_double)]
+
+
 class Beamlet(ctypes.Structure):
     _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("particle", ctypes.c_int),
                 ("energy", ctypes.c_double), ("nhist", ctypes.c_uint64)]
@@ -128,6 +134,8 @@ def lib():
         L.dpm_host_free.argtypes = [vp]
         L.dpm_probe_atomics.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp]
         L.dpm_phantom_set_majorant.argtypes = [vp, dp]
+        L.dpm_simulate_phsp.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(RunOptions), vp, vp,
+                                        ctypes.POINTER(Stats)]
         L.dpm_simulate_noalloc.argtypes = [vp, vp, ctypes.POINTER(Source), ctypes.c_uint64,
                                            ctypes.POINTER(RunOptions), vp, vp, ctypes.POINTER(Stats)]
         L.dpm_phantom_get_majorant.argtypes = [vp]
@@ -318,6 +326,41 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
         tally = tally.reshape(phantom.shape)
         steps = steps.reshape(phantom.shape)
     return tally, steps, st.as_dict(), he
+
+
+def phsp_records(rows):
+    """Phase-space records as the C array: rows of (particle, E, x, y, z, u, v, w)."""
+    rows = np.asarray(rows, dtype=np.float64).reshape(-1, 8)
+    rec = np.zeros(rows.shape[0], dtype=np.dtype([("particle", "<i4"), ("pad", "<i4"), ("energy", "<f8"),
+                                                  ("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("u", "<f8"),
+                                                  ("v", "<f8"), ("w", "<f8")]))
+    assert rec.dtype.itemsize == ctypes.sizeof(PhspParticle)
+    rec["particle"] = rows[:, 0].astype(np.int32)
+    for j, f in enumerate(("energy", "x", "y", "z", "u", "v", "w")):
+        rec[f] = rows[:, 1 + j]
+    return rec
+
+
+def simulate_phsp(tables, phantom, rows, seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, schedule=None,
+                  per_history=False):
+    """dpm_simulate_phsp: one history per phase-space record, rows of
+    (particle, E, x, y, z, u, v, w). Returns (tally, steps_or_sum2, stats dict,
+    per-history array or None); history ids put the photon records first."""
+    rec = phsp_records(rows)
+    n = rec.shape[0]
+    he = np.zeros(n) if per_history else None
+    opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, hist_edep=_ptr(he))
+    if schedule is not None:
+        opt.chunk, opt.waves_per_cu = schedule["chunk"], schedule["waves_per_cu"]
+        opt.refill_min, opt.tally_mode = schedule["refill_min"], schedule["tally_mode"]
+        opt.pipes = schedule.get("pipes", 0)
+    tally, second = np.zeros(phantom.nbins), np.zeros(phantom.nbins)
+    st = Stats()
+    _check(lib().dpm_simulate_phsp(tables.h, phantom.h, rec.ctypes.data_as(ctypes.c_void_p), n, ctypes.byref(opt),
+                                   _ptr(tally), _ptr(second), ctypes.byref(st)))
+    if phantom.shape is not None:
+        tally, second = tally.reshape(phantom.shape), second.reshape(phantom.shape)
+    return tally, second, st.as_dict(), he
 
 
 def simulate_noalloc(tables, phantom, nhist, d_tally, d_steps=None, particle=ELECTRON, energy=15.0, emin=0.0,

@@ -288,6 +288,32 @@ int dpm_simulate_fields(dpm_tables* t, const dpm_phantom* p, const dpm_source* s
 int dpm_simulate_noalloc(dpm_tables* t, const dpm_phantom* p, const dpm_source* src, uint64_t nhist,
                          const dpm_run_options* opt, double* d_tally, double* d_steps, dpm_stats* stats);
 
+/* Phase-space source: a list of particles (a phase-space file, e.g. scored
+ * below a treatment head), each one history that starts where and as the
+ * record says -- no counterpart in the reference, whose source is the fixed
+ * pencil of SimDPMLike.cc:64-75 (one primary per history at a given position,
+ * direction and energy; a record generalises it). The tally is returned as by
+ * dpm_simulate (with nbatch > 1 on voxel phantoms, `steps` receives the sum
+ * of squares). History ids: the photon records first, then the e-/e+
+ * records, each in the order given, from opt->hist_begin; each history's
+ * random stream starts unused (no source draws). Positions are phantom
+ * coordinates (voxel phantoms: centred on the origin; slab: the reference's
+ * Geom frame). In a voxel phantom a record outside the phantom moves along its
+ * direction to where it enters (1e-3 mm inside); a record whose ray misses the
+ * phantom, or a slab record outside the slab's extent, leaves at once (its
+ * energy counts as escaped). Directions must be unit vectors (within 1e-6),
+ * energies above the tracking cut of the particle and below the tables'
+ * maximum; at most 2^30 records per call. */
+typedef struct dpm_phsp_particle {
+  int particle;                    /* DPM_ELECTRON, DPM_PHOTON or DPM_POSITRON */
+  double energy;                   /* MeV */
+  double x, y, z;                  /* mm */
+  double u, v, w;                  /* direction cosines */
+} dpm_phsp_particle;
+
+int dpm_simulate_phsp(dpm_tables* t, const dpm_phantom* p, const dpm_phsp_particle* rec, uint64_t n,
+                      const dpm_run_options* opt, double* tally, double* steps, dpm_stats* stats);
+
 /* Copies the tally of the last dpm_simulate on this handle (and, if
  * `second` is not NULL, its steps / sum-of-squares array) into host arrays of
  * dpm_phantom_num_bins elements. No counterpart in the reference (which writes

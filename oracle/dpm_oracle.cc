@@ -1107,6 +1107,27 @@ bool gantry_entry(double fu, double fv, double ca, double sa, const double h[3],
   return true;
 }
 
+// a phase-space record outside the voxel phantom (its faces included) moves
+// along its direction to where it enters the phantom, 1e-3 mm inside; false
+// if the ray misses
+bool phsp_entry(const double h[3], double pos[3], const double dir[3]) {
+  if (std::fabs(pos[0]) <= h[0] && std::fabs(pos[1]) <= h[1] && std::fabs(pos[2]) <= h[2]) return true;
+  double tin = -1e30, tout = 1e30;
+  for (int i = 0; i < 3; ++i) {
+    if (dir[i] == 0.0) {
+      if (pos[i] <= -h[i] || pos[i] >= h[i]) return false;
+      continue;
+    }
+    const double t1 = (-h[i] - pos[i]) / dir[i], t2 = (h[i] - pos[i]) / dir[i];
+    tin = std::max(tin, std::min(t1, t2));
+    tout = std::min(tout, std::max(t1, t2));
+  }
+  if (!(tin >= 0.0 && tin + 2e-3 < tout)) return false;
+  const double t = tin + 1e-3;
+  for (int i = 0; i < 3; ++i) pos[i] = pos[i] + t * dir[i];
+  return true;
+}
+
 // the state of a primary as generated (x y z u v w E)
 void store_primary(const Track& p, double* o) {
   for (int j = 0; j < 3; ++j) { o[j] = p.pos[j]; o[3 + j] = p.dir[j]; }
@@ -1191,7 +1212,22 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
     p.pos[0] = cfg->x0; p.pos[1] = cfg->y0; p.pos[2] = zstart;
     p.dir[0] = 0.0; p.dir[1] = 0.0; p.dir[2] = 1.0;
     p.ekin = cfg->e0;
-    if (cfg->rng_mode == ORACLE_RNG_PHILOX) {
+    if (cfg->src_kind == ORACLE_SRC_PHSP) {
+      const double* r = cfg->phsp + 8 * ih;
+      p.type = (int)r[0];
+      p.ekin = r[1];
+      for (int j = 0; j < 3; ++j) { p.pos[j] = r[2 + j]; p.dir[j] = r[5 + j]; }
+      ep += p.ekin;
+      if (geo.mode == ORACLE_GEOM_VOXEL) {
+        const double h[3] = {-geo.org[0], -geo.org[1], -geo.org[2]};
+        if (!phsp_entry(h, p.pos, p.dir)) {
+          s.escaped += p.ekin;
+          if (out->hist_edep) out->hist_edep[ih] = 0.0;
+          if (out->prim) store_primary(p, out->prim + 7 * ih);
+          continue;
+        }
+      }
+    } else if (cfg->rng_mode == ORACLE_RNG_PHILOX) {
       s.rng.cur = &p;
       if (cfg->src_kind == ORACLE_SRC_FIELD || cfg->src_kind == ORACLE_SRC_GANTRY) {
         const double ux = s.rng.u(), uy = s.rng.u();

@@ -26,7 +26,7 @@ extern "C" {
 
 enum { ORACLE_RNG_MT19937 = 0, ORACLE_RNG_PHILOX = 1 };
 enum { ORACLE_GEOM_SLAB = 0, ORACLE_GEOM_VOXEL = 1 };
-enum { ORACLE_SRC_PENCIL = 0, ORACLE_SRC_FIELD = 1, ORACLE_SRC_GANTRY = 2 };
+enum { ORACLE_SRC_PENCIL = 0, ORACLE_SRC_FIELD = 1, ORACLE_SRC_GANTRY = 2, ORACLE_SRC_PHSP = 3 };
 enum { ORACLE_SPEC_FLAT = 0, ORACLE_SPEC_TRIANGULAR = 1, ORACLE_SPEC_TABLE = 2 };
 
 typedef struct oracle_cfg {
@@ -58,6 +58,10 @@ typedef struct oracle_cfg {
   int      spec_n;
   const double* spec_edges;   /* [spec_n + 1] */
   const double* spec_weights; /* [spec_n] */
+  /* ORACLE_SRC_PHSP: [nhist][8] records (particle -1/0/+1, E, x, y, z, u, v, w);
+     history ih is record ih (id hist_begin + ih), no source draws; in a voxel
+     phantom a record outside moves along its direction to where it enters */
+  const double* phsp;
 } oracle_cfg;
 
 typedef struct oracle_out {

@@ -14,7 +14,7 @@ CLI_PATH = os.path.join(ROOT, "oracle", "build", "dpm_oracle_cli")
 
 RNG_MT, RNG_PHILOX = 0, 1
 GEOM_SLAB, GEOM_VOXEL = 0, 1
-SRC_PENCIL, SRC_FIELD, SRC_GANTRY = 0, 1, 2
+SRC_PENCIL, SRC_FIELD, SRC_GANTRY, SRC_PHSP = 0, 1, 2, 3
 SPEC_FLAT, SPEC_TRIANGULAR, SPEC_TABLE = 0, 1, 2
 
 NCNT = 9
@@ -33,7 +33,8 @@ class Cfg(ctypes.Structure):
                 ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double),
                 ("hist_begin", ctypes.c_uint64), ("nhist", ctypes.c_uint64), ("nbatch", ctypes.c_int),
                 ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double),
-                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p)]
+                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p),
+                ("phsp", ctypes.c_void_p)]
 
 
 class Out(ctypes.Structure):
@@ -103,11 +104,16 @@ def _p(a):
 
 
 def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_MT, seed=123,
-             arg_rtl=1, hist_begin=0, per_history=False, hist_cap=4096):
+             arg_rtl=1, hist_begin=0, per_history=False, hist_cap=4096, phsp=None):
+    """Slab run; phsp: optional [nhist, 8] records (particle, E, x, y, z, u, v, w), one per history."""
     load(datadir)
     cfg = Cfg(rng_mode=rng, arg_rtl=arg_rtl, seed=seed, geom_mode=GEOM_SLAB, lbox=lbox, geom_index=geom_index,
               majorant_scale=1.0, src_kind=SRC_PENCIL, ptype=ptype, e0=e0, emin=0.0, emax=0.0,
               hist_begin=hist_begin, nhist=nhist)
+    if phsp is not None:
+        phsp = np.ascontiguousarray(phsp, dtype=np.float64)
+        assert phsp.shape == (nhist, 8)
+        cfg.src_kind, cfg.phsp = SRC_PHSP, _p(phsp)
     edep = np.zeros(hist_cap)
     steps = np.zeros(hist_cap)
     hed = np.zeros(nhist) if per_history else None
@@ -121,13 +127,14 @@ def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_
 
 def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
               seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None,
-              spectrum_table=None, field_y=None, primaries=False):
+              spectrum_table=None, field_y=None, primaries=False, phsp=None):
     """Voxel run in Philox mode. angle (degrees) selects the gantry source (a
     field, or a pencil when field == 0, whose ray enters the phantom from the
     side the beam direction (sin a, 0, cos a) points away from); emode selects
     the triangular spectrum (emin, emode, emax); field_y (default: field) is
     the field's y size; primaries=True returns each primary as generated
-    (x y z u v w E per history) under "prim"."""
+    (x y z u v w E per history) under "prim"; phsp: [nhist, 8] phase-space
+    records (particle, E, x, y, z, u, v, w), one per history."""
     load(datadir)
     mat = np.ascontiguousarray(mat, dtype=np.uint8)
     rho = np.ascontiguousarray(rho, dtype=np.float32)
@@ -139,6 +146,10 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
               emode=0.0 if emode is None else emode, ptype=ptype, e0=e0, emin=emin, emax=emax,
               fx=field, fy=field if field_y is None else field_y, x0=x0, y0=y0, hist_begin=hist_begin,
               nhist=nhist, nbatch=nbatch)
+    if phsp is not None:
+        phsp = np.ascontiguousarray(phsp, dtype=np.float64)
+        assert phsp.shape == (nhist, 8)
+        cfg.src_kind, cfg.phsp = SRC_PHSP, _p(phsp)
     if spectrum_table is not None:
         edges = np.ascontiguousarray(spectrum_table[0], dtype=np.float64)
         weights = np.ascontiguousarray(spectrum_table[1], dtype=np.float64)

@@ -38,7 +38,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     # sizes and member offsets of the ctypes mirrors against the C compiler's
     # view of include/dpm_amd.h
     structs = {"dpm_stats": dpm_amd.Stats, "dpm_run_options": dpm_amd.RunOptions, "dpm_source": dpm_amd.Source,
-               "dpm_schedule": dpm_amd.Schedule, "dpm_beamlet": dpm_amd.Beamlet, "dpm_field": dpm_amd.Field}
+               "dpm_schedule": dpm_amd.Schedule, "dpm_beamlet": dpm_amd.Beamlet, "dpm_field": dpm_amd.Field,
+               "dpm_phsp_particle": dpm_amd.PhspParticle}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dpm_amd.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
@@ -128,3 +129,14 @@ def test_voxeldose_rejects_a_malformed_spectrum_line(tmp_path, tables0, text):
                         "-n", "10", "-o", str(tmp_path / "x.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "expected 'E_low E_high weight'" in r.stderr
+
+
+@pytest.mark.parametrize("text", ["gamma 5 0 0 -10 0 0 1\ne- 5 0 0 -10 0 0\n", "muon 5 0 0 -10 0 0 1\n",
+                                  "gamma 5 0 0 -10 0 0 1 9\n"])
+def test_voxeldose_rejects_a_malformed_phase_space_line(tmp_path, tables0, text):
+    f = tmp_path / "ps.txt"
+    f.write_text("# particle E x y z u v w\n" + text)
+    r = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", "4", "-y", "4", "-z", "4", "-P", str(f),
+                        "-o", str(tmp_path / "x.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "expected 'particle E x y z u v w'" in r.stderr

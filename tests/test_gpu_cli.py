@@ -143,3 +143,36 @@ def test_dpm_voxeldose_cli_spectrum_file(gpu_tables0, tables0, tmp_path):
     r = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", "4", "-y", "4", "-z", "4", "-S", str(bad),
                         "-n", "10", "-o", str(tmp_path / "x.bin")], capture_output=True, timeout=60)
     assert r.returncode != 0
+
+
+@pytest.mark.gpu
+def test_dpm_voxeldose_cli_phase_space_file(gpu_tables0, tables0, tmp_path):
+    # -P phase-space file: one history per record; the dose equals the
+    # library's dpm_simulate_phsp on the same records
+    nx = ny = nz = 12
+    lbox = 2.0
+    rng = np.random.default_rng(4)
+    n = 3000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    kinds = rng.choice(["gamma", "e-", "e+"], n)
+    rows = np.column_stack([rng.uniform(1, 10, n), rng.uniform(-10, 10, (n, 3)), d])
+    lines = ["# particle E x y z u v w"] + ["%s %.17g %.17g %.17g %.17g %.17g %.17g %.17g" % ((k,) + tuple(r))
+                                            for k, r in zip(kinds, rows)]
+    ps = tmp_path / "ps.txt"
+    ps.write_text("\n".join(lines) + "\n")
+    out = tmp_path / "dose.bin"
+    subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", str(nx), "-y", str(ny), "-z", str(nz), "-l",
+                    str(lbox), "-P", str(ps), "-k", "5", "-s", "7", "-f", "-o", str(out)], check=True,
+                   capture_output=True, timeout=120)
+    v = dpm_amd.read_voxel_dose(str(out))
+    assert v["nhist"] == n
+    mat = np.zeros((nz, ny, nx), np.uint8)
+    rho = np.ones((nz, ny, nx), np.float32)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, lbox)
+    code = {"gamma": 0, "e-": -1, "e+": 1}
+    recs = np.column_stack([[code[k] for k in kinds], rows])
+    edep, _, _, _ = dpm_amd.simulate_phsp(gpu_tables0, ph, recs, seed=7, nbatch=5, precision=dpm_amd.DPM_FP64)
+    mass_kg = rho.astype(np.float64) * (lbox * 0.1) ** 3 * 1e-3
+    np.testing.assert_allclose(v["dose"], edep * MEV_J / mass_kg / n, rtol=1e-9, atol=1e-30)
+    ph.close()
