@@ -35,15 +35,22 @@ brem IMFP on [max(ecut,gcut), Emax] and Moller IMFP on [2 ecut, Emax]
 GS 256 x 64 on [ecut, Emax], photon data 1024 *linear* points on
 [0.99 gcut, Emax] (``PhotonData.hh:60-71``).
 
-Physics models (all written here from the cited textbook formulae, not from
+Physics models (written here from the cited formulae and data, not from
 Geant4):
 
 * elastic: screened-Rutherford (Wentzel) DCS per element with Moliere's
   screening parameter; exact transport coefficients of that DCS.
 * collision stopping power: Berger-Seltzer restricted formula (ICRU 37) with
-  the Sternheimer-Peierls general density-effect parametrisation.
-* bremsstrahlung: Bethe-Heitler DCS, smallest of the unscreened and the
-  complete-screening logarithm (Koch & Motz 1959, formulae 3BN limits).
+  Sternheimer's density-effect parameters (tabulated x0, x1, m, delta0 of
+  Sternheimer, Berger & Seltzer 1984; general Sternheimer-Peierls rules for
+  other materials).
+* bremsstrahlung: the Seltzer-Berger scaled DCS (the reference's
+  SBTables/data/NIST_BREM values of the elements used, kept in
+  sb_nist_brem.npz): sampling tables by the reference's own construction
+  (SBTableBuilder.cc:69-241), and the brem cross section and restricted
+  radiative stopping power as integrals of the same spline DCS. The
+  Bethe-Heitler closed form (Koch & Motz 1959, 3BN limits) is kept as a
+  cross-check.
 * Moller: Moller DCS and its closed-form restricted integrals.
 * photons: Klein-Nishina (Compton), Bethe-Heitler pair with a threshold
   factor, and a Z^4.6 E^-3.5 photo-absorption fit normalised to water.
@@ -52,7 +59,8 @@ Geant4):
   (two-or-more scattering) part, transformed with mu(u)=1-2au/(1-u+a) and
   tabulated for rational-interpolation (RITA) inversion.
 
-The tables are therefore physically plausible but not Geant4's; GPU-vs-CPU
+The tables are therefore close to, but not, Geant4's (the reference test
+case's depth dose agrees with the Geant4-table run at 1 %/1 mm); GPU-vs-CPU
 parity is checked on identical files written by this module.
 """
 import argparse
@@ -213,21 +221,47 @@ def find_screening(g1):
 # --------------------------------------------------------------------------
 # stopping power
 # --------------------------------------------------------------------------
+# Sternheimer density-effect parameters (Sternheimer, Berger & Seltzer, At.
+# Data Nucl. Data Tables 30 (1984) 261): the table's I [eV] and Cbar (for the
+# table's own I and density; density_effect recomputes Cbar for the
+# material's), x0, x1, m, delta0
+STERNHEIMER = {
+    "G4_WATER": (75.0, 3.5017, 0.2400, 2.8004, 3.4773, 0.0),
+    "G4_AIR": (85.7, 10.5961, 1.7418, 4.2759, 3.3994, 0.0),
+    "G4_BONE_CORTICAL_ICRP": (106.4, 3.6488, 0.1161, 3.0919, 3.5919, 0.0),
+    "G4_Ti": (233.0, 4.4450, 0.0957, 3.0386, 3.0679, 0.12),
+    "G4_BONE_COMPACT_ICRU": (91.9, 3.3390, 0.0944, 3.0201, 3.6419, 0.0),
+    "G4_W": (727.0, 5.4059, 0.2167, 3.4960, 2.8447, 0.14),
+}
+
+
+def plasma_energy(mat):
+    """hbar omega_p = 28.816 eV sqrt(rho Z/A) [MeV]."""
+    return 28.816e-6 * math.sqrt(mat.rho * mat.Zbar / mat.Abar)
+
+
 def density_effect(mat, t):
-    """Sternheimer-Peierls general parametrisation of delta(beta gamma)."""
-    hw_p = 28.816e-6 * math.sqrt(mat.rho * mat.Zbar / mat.Abar)      # [MeV]
-    cbar = 1.0 + 2.0 * math.log(mat.I / hw_p)
-    if mat.rho < 0.01:  # gases
-        x0, x1 = (1.6, 4.0) if cbar < 10.0 else (0.326 * cbar - 1.5, 4.0)
-    elif mat.I < 100e-6:
-        x0, x1 = (0.2, 2.0) if cbar < 3.681 else (0.326 * cbar - 1.0, 2.0)
-    else:
-        x0, x1 = (0.2, 3.0) if cbar < 5.215 else (0.326 * cbar - 1.5, 3.0)
-    m = 3.0
-    a = (cbar - 4.6052 * x0) / (x1 - x0) ** m
+    """delta(beta gamma) of the Sternheimer parametrisation: 0 (insulators) or
+    delta0 10^(2(x - x0)) (conductors) below x0 = log10(p/mc), 4.6052 x - Cbar
+    + a (x1 - x)^m up to x1, 4.6052 x - Cbar above, with Cbar = 1 + 2
+    ln(I/hbar omega_p) of the material's own I and density and a from
+    continuity at x0. Tabulated materials use their published x0, x1, m and
+    delta0; the others the general Sternheimer-Peierls rules."""
     pc, _, _ = kinematics(t)
     x = np.log10(pc / EMC2)
-    d = np.where(x < x0, 0.0, 4.6052 * x - cbar + a * np.clip(x1 - x, 0.0, None) ** m)
+    cbar = 1.0 + 2.0 * math.log(mat.I / plasma_energy(mat))
+    if mat.name in STERNHEIMER:
+        _, _, x0, x1, m, d0 = STERNHEIMER[mat.name]
+    else:
+        d0, m = 0.0, 3.0
+        if mat.rho < 0.01:  # gases
+            x0, x1 = (1.6, 4.0) if cbar < 10.0 else (0.326 * cbar - 1.5, 4.0)
+        elif mat.I < 100e-6:
+            x0, x1 = (0.2, 2.0) if cbar < 3.681 else (0.326 * cbar - 1.0, 2.0)
+        else:
+            x0, x1 = (0.2, 3.0) if cbar < 5.215 else (0.326 * cbar - 1.5, 3.0)
+    a = (cbar - 4.6052 * x0 + d0) / (x1 - x0) ** m
+    d = np.where(x < x0, d0 * 10.0 ** (2.0 * (x - x0)), 4.6052 * x - cbar + a * np.clip(x1 - x, 0.0, None) ** m)
     return np.where(x >= x1, 4.6052 * x - cbar, d)
 
 
@@ -250,7 +284,8 @@ def restricted_collision_dedx(mat, t, ecut):
 # --------------------------------------------------------------------------
 def brem_kdsdk(mat, t, k):
     """k dsigma/dk per gram [cm2/g] (Bethe-Heitler, screening by the smaller
-    logarithm). t, k broadcast together, 0 < k <= t."""
+    logarithm). t, k broadcast together, 0 < k <= t. A cross-check of the
+    Seltzer-Berger values the tables are built from."""
     e0 = t + EMC2
     e = e0 - k
     v = e / e0
@@ -284,6 +319,174 @@ def brem_integrals(mat, t, gcut):
     lk = lo + (hi - lo)[:, None] * x[None, :]
     sig = np.sum(w[None, :] * brem_kdsdk(mat, t[:, None], np.exp(lk)), axis=1) * (hi - lo)
     sig = np.where(t > gcut, sig, 0.0)
+    return srad, sig
+
+
+# Seltzer-Berger scaled DCS chi(Z, T, kappa) = (beta^2/Z^2) k dsigma/dk [mb]
+# of the ELEMENTS, on the NIST grid of 113 kinetic energies x 32 reduced
+# photon energies kappa = k/T (written by make_sb_data.py from the reference's
+# SBTables/data/NIST_BREM files)
+SB_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sb_nist_brem.npz")
+MBARN_CM2 = 1.0e-27
+_SB = None
+
+
+def sb_data():
+    """(energy[113] MeV, kappa[32], {Z: chi[113, 32] in mb})."""
+    global _SB
+    if _SB is None:
+        with np.load(SB_DATA, allow_pickle=False) as d:
+            _SB = (d["energy"].copy(), d["kappa"].copy(),
+                   {int(z): d["chi"][i].copy() for i, z in enumerate(d["Z"])})
+    return _SB
+
+
+class NaturalSpline:
+    """Natural cubic spline (zero second derivative at both ends) through
+    (x_i, y_i) with strictly monotone x, increasing or decreasing.
+
+    Evaluation follows Utils/src/Spline.cc:73-108: the segment is found by
+    bisection and clamped to the end segments outside [x_0, x_{n-1}], and a
+    segment whose two node values sum to zero gives zero. The coefficients are
+    the textbook ones (second derivatives from the tridiagonal continuity
+    system of the first derivative, solved by forward elimination)."""
+
+    def __init__(self, x, y):
+        x = np.asarray(x, dtype=float)
+        y = np.asarray(y, dtype=float)
+        n = len(x)
+        if n < 2 or len(y) != n:
+            raise ValueError("spline needs >= 2 matching nodes")
+        h = np.diff(x)
+        if not (np.all(h > 0) or np.all(h < 0)):
+            raise ValueError("spline nodes must be strictly monotone")
+        s = np.diff(y) / h
+        m2 = np.zeros(n)
+        if n > 2:
+            diag = 2.0 * (h[:-1] + h[1:])
+            rhs = 6.0 * (s[1:] - s[:-1])
+            for i in range(1, n - 2):
+                f = h[i] / diag[i - 1]
+                diag[i] -= f * h[i]
+                rhs[i] -= f * rhs[i - 1]
+            m2[n - 2] = rhs[-1] / diag[-1]
+            for i in range(n - 4, -1, -1):
+                m2[i + 1] = (rhs[i] - h[i + 1] * m2[i + 2]) / diag[i]
+        self.x, self.y, self.m2 = x, y, m2
+        self.b = s - h * (2.0 * m2[:-1] + m2[1:]) / 6.0
+        self.c = 0.5 * m2[:-1]
+        self.d = (m2[1:] - m2[:-1]) / (6.0 * h)
+        self.increasing = h[0] > 0
+
+    def segment(self, v):
+        """Index m of the segment that holds v (clamped to 0 .. n-2)."""
+        x, n = self.x, len(self.x)
+        v = np.asarray(v, dtype=float)
+        if self.increasing:
+            m = np.searchsorted(x[:-1], v, side="left")
+            m = np.where(m > 0, m - 1, 0)
+        else:
+            m = n - np.searchsorted(x[::-1], v, side="right") - 1
+        return np.clip(m, 0, n - 2)
+
+    def __call__(self, v):
+        m = self.segment(v)
+        t = np.asarray(v, dtype=float) - self.x[m]
+        val = self.y[m] + t * (self.b[m] + t * (self.c[m] + t * self.d[m]))
+        return np.where(self.y[m] + self.y[m + 1] == 0.0, 0.0, val)
+
+
+def sb_phi(kappa):
+    """phi(kappa) = ln(1 - kappa + 1e-12), the spline variable of the scaled
+    DCS (SBTableBuilder.cc:146-148): it spreads the tip kappa -> 1."""
+    return np.log(1.0 - np.asarray(kappa, dtype=float) + 1.0e-12)
+
+
+def sb_material_dcs(mat, t, sb=None):
+    """X_j = sum_i n_i Z_i^2 chi_i(T, kappa_j) [cm^2/g] at the 32 kappa nodes,
+    n_i in atoms per gram. chi_i is interpolated linearly in (ln T, ln chi)
+    between the grid energies E_e <= T < E_e+1, per kappa node
+    (SBTableBuilder.cc:131-160)."""
+    energy, kappa, chi = sb or sb_data()
+    e = int(np.searchsorted(energy, t, side="right")) - 1
+    if not 0 <= e < len(energy) - 1:
+        raise ValueError("T = %g MeV outside the scaled-DCS grid" % t)
+    dum0 = math.log(t / energy[e])
+    dum1 = math.log(energy[e + 1] / energy[e])
+    x = np.zeros(len(kappa))
+    for z, npg in zip(mat.Z, mat.atoms_per_gram):
+        lo = chi[int(round(z))][e] * MBARN_CM2
+        hi = chi[int(round(z))][e + 1] * MBARN_CM2
+        x += np.exp(np.log(hi / lo) / dum1 * dum0 + np.log(lo)) * npg * z * z
+    return x
+
+
+def sb_alias_points(mat, gcut, t, ns=64, sb=None):
+    """The ns (xi, p(xi)) points of one SB sampling table at kinetic energy T,
+    restating SBTableBuilder::BuildOneLinAlias (SBTableBuilder.cc:120-241):
+    xi(kappa) = 1 - ln(kappa)/ln(kappa_c), kappa_c = gcut/T, spans the photon
+    energies gcut..T; the points are xi = 0 (spline value at kappa_c), the
+    loaded nodes kappa_c < kappa_k < 1 with their node values, xi = 1 with the
+    kappa = 1 value, then, until ns points, the midpoint of the interval whose
+    linear interpolation is furthest from the spline, with its spline value.
+    p is in the reference's per-volume unit (atoms/mm^3 x mm^2)."""
+    sb = sb or sb_data()
+    kappa = sb[1]
+    nk = len(kappa)
+    dcs = sb_material_dcs(mat, t, sb) * 0.1 * mat.rho
+    sp = NaturalSpline(sb_phi(kappa), dcs)
+    kc = gcut / t
+    lkc = math.log(kc)
+    ki = 0
+    while ki < nk - 1 and kc >= kappa[ki]:
+        ki += 1
+    ki -= 1
+    xs = [0.0]
+    ys = [float(sp(sb_phi(kc)))]
+    for k in range(ki + 1, nk - 1):
+        xs.append(1.0 - math.log(kappa[k]) / lkc)
+        ys.append(float(dcs[k]))
+    xs.append(1.0)
+    ys.append(float(dcs[nk - 1]))
+    while len(xs) < ns:
+        xa, ya = np.array(xs), np.array(ys)
+        xm = 0.5 * (xa[:-1] + xa[1:])
+        lin = 0.5 * (ya[:-1] + ya[1:])
+        spv = sp(sb_phi(kc * np.exp(-xm * lkc)))
+        j = int(np.argmax(np.abs(lin - spv)))
+        xs.insert(j + 1, float(xm[j]))
+        ys.insert(j + 1, float(spv[j]))
+    return np.array(xs), np.array(ys)
+
+
+def sb_brem_integrals(mat, t, gcut, sb=None, ngl=16):
+    """Restricted radiative stopping power S_rad(T) = int_0^min(gcut,T) k
+    dsigma/dk dk [MeV cm^2/g] and the brem cross section sigma(T) =
+    int_gcut^T dsigma/dk dk [cm^2/g] from the scaled DCS: per atom
+    k dsigma/dk = Z^2 chi(kappa)/beta^2, so with X the spline of
+    sb_material_dcs in phi(kappa) and kappa_c = min(gcut/T, 1),
+    S_rad = T/beta^2 int_0^kappa_c X dkappa, sigma = 1/beta^2
+    int_kappa_c^1 X dln(kappa). Gauss-Legendre (ngl points) on every interval
+    between the kappa nodes."""
+    sb = sb or sb_data()
+    kappa = sb[1]
+    x, w = _gl(ngl)
+    t = np.atleast_1d(np.asarray(t, dtype=float))
+    srad = np.zeros(len(t))
+    sig = np.zeros(len(t))
+    for i, ti in enumerate(t):
+        sp = NaturalSpline(sb_phi(kappa), sb_material_dcs(mat, ti, sb))
+        beta2 = ti * (ti + 2.0 * EMC2) / (ti + EMC2) ** 2
+        kc = min(gcut / ti, 1.0)
+        nodes = np.concatenate([[0.0], kappa[(kappa > 0.0) & (kappa < kc)], [kc]])
+        a, b = nodes[:-1, None], nodes[1:, None]
+        kk = a + (b - a) * x[None, :]
+        srad[i] = np.sum((b - a) * w[None, :] * sp(sb_phi(kk))) * ti / beta2
+        if kc < 1.0:
+            nodes = np.log(np.concatenate([[kc], kappa[(kappa > kc) & (kappa < 1.0)], [1.0]]))
+            a, b = nodes[:-1, None], nodes[1:, None]
+            uu = a + (b - a) * x[None, :]
+            sig[i] = np.sum((b - a) * w[None, :] * sp(sb_phi(np.exp(uu)))) / beta2
     return srad, sig
 
 
@@ -547,7 +750,7 @@ def generate(outdir, index, verbose=True):
         lam_el = 1.0 / (sig_el * m.rho) * 10.0          # mm
         lam_tr = 1.0 / (sig_tr * m.rho) * 10.0          # mm
         scr = find_screening(sig_tr / sig_el)
-        srad, _ = brem_integrals(m, egrid, gcut)
+        srad, _ = sb_brem_integrals(m, egrid, gcut)
         dedx = restricted_collision_dedx(m, egrid, ecut) + srad   # MeV cm2/g
         fx = np.arctan(5.0 * np.log(egrid / cfg.e_cross)) / math.pi + 0.5
         smax = cfg.s_low * (cfg.s_high / cfg.s_low) ** fx
@@ -674,7 +877,7 @@ def generate(outdir, index, verbose=True):
         f.write("# number of data per materials and #materials\n %d %d\n" % (len(bgrid), nmat))
         f.write("# -----------------------------\n")
         for im, m in enumerate(mats):
-            _, sig = brem_integrals(m, bgrid, gcut)
+            _, sig = sb_brem_integrals(m, bgrid, gcut)
             sig = np.maximum(sig * 0.1, 1e-20 / m.rho)
             f.write("#\n# Material name: %s\n#\n" % m.name)
             for ie, e in enumerate(bgrid):
@@ -733,10 +936,7 @@ def generate(outdir, index, verbose=True):
             f.write("#\n# Material name: %s\n#\n" % m.name)
             for ie, e in enumerate(sgrid):
                 ep = e + (1e-6 if ie == 0 else (-1e-6 if ie == len(sgrid) - 1 else 0.0))
-                k = gcut * np.exp(xi64 * np.log(ep / gcut))
-                k = np.minimum(k, ep)
-                y = brem_kdsdk(m, np.full_like(k, ep), k) * 1e27
-                write_alias(f, xi64, y)
+                write_alias(f, *sb_alias_points(m, gcut, ep))
 
     # ---- photons
     pgrid = np.empty(cfg.n_ekin_ph)

@@ -1,9 +1,17 @@
 """Checks of the Geant4-free table generator: the alias tables sample the
 tabulated linear pdf exactly, the GS RITA tables invert their distribution,
-and key physics values sit near published data (NIST XCOM / ESTAR)."""
+the Seltzer-Berger tables follow the reference's construction on its NIST
+data, and key physics values sit near published data (NIST XCOM / ESTAR,
+Sternheimer's density-effect tables)."""
+import math
+import os
+
 import numpy as np
+import pytest
 
 import dpm_tablegen as tg
+
+NIST_BREM = "/root/reference/SBTables/data/NIST_BREM"
 
 
 def _alias_bin_probabilities(w, alias):
@@ -108,4 +116,106 @@ def test_physics_values_near_published_data():
     assert abs(mu[1] - 0.0222) / 0.0222 < 0.03      # XCOM, 10 MeV
     s = tg.restricted_collision_dedx(w, np.array([1.0, 10.0]), np.array([1.0, 10.0]))
     assert abs(s[0] - 1.849) / 1.849 < 0.02         # ESTAR collision stopping power
-    assert abs(s[1] - 1.968) / 1.968 < 0.02
+    assert abs(s[1] - 1.968) / 1.968 < 0.01         # (ESTAR takes I = 75 eV, G4 78 eV)
+    r, _ = tg.sb_brem_integrals(w, np.array([1.0, 10.0]), 1.0e9)
+    assert abs(r[0] - 0.01293) / 0.01293 < 0.03     # ESTAR radiative stopping power
+    assert abs(r[1] - 0.1832) / 0.1832 < 0.02
+
+
+def test_natural_spline_matches_scipy():
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(5)
+    x = np.sort(rng.uniform(-5.0, 5.0, 20))
+    y = rng.normal(size=20)
+    ref = CubicSpline(x, y, bc_type="natural")       # extrapolates with the end cubics
+    q = np.concatenate([np.linspace(x[0] - 0.5, x[-1] + 0.5, 1001), x])
+    for sp in (tg.NaturalSpline(x, y), tg.NaturalSpline(x[::-1], y[::-1])):
+        assert np.allclose(sp(q), ref(q), rtol=0.0, atol=1e-9)
+        assert np.allclose(sp(x), y, rtol=0.0, atol=1e-12)
+    assert tg.NaturalSpline([0.0, 1.0, 2.0], [0.0, 0.0, 1.0])(0.5) == 0.0   # zero segment rule
+
+
+def test_sb_dcs_is_log_log_in_energy_and_sums_z_squared():
+    energy, kappa, chi = tg.sb_data()
+    ti = tg.Material("G4_Ti")
+    f = 1e-27 * ti.atoms_per_gram[0] * 22 * 22
+    for i in (40, 70, 100):
+        assert np.allclose(tg.sb_material_dcs(ti, energy[i]), chi[22][i] * f, rtol=1e-12, atol=0.0)
+        mid = tg.sb_material_dcs(ti, math.sqrt(energy[i] * energy[i + 1]))
+        assert np.allclose(mid, np.sqrt(chi[22][i] * chi[22][i + 1]) * f, rtol=1e-10, atol=0.0)
+    w = tg.Material("G4_WATER")
+    i = 60
+    want = sum(n * z * z * chi[int(z)][i] * 1e-27 for z, n in zip(w.Z, w.atoms_per_gram))
+    assert np.allclose(tg.sb_material_dcs(w, energy[i]), want, rtol=1e-12, atol=0.0)
+
+
+@pytest.mark.parametrize("t", [0.05 + 1e-6, 0.3, 1.0, 15.0, 21.0 - 1e-6])
+def test_sb_alias_points_follow_the_builder(t):
+    """SBTableBuilder.cc:161-229: xi = 0 at kappa_c, every loaded kappa node in
+    (kappa_c, 1) with its node value, xi = 1 with the kappa = 1 value, the rest
+    on the spline; the linear pdf then carries the spline DCS bin by bin."""
+    _, kappa, _ = tg.sb_data()
+    m, gcut = tg.Material("G4_WATER"), 0.05
+    x, y = tg.sb_alias_points(m, gcut, t)
+    assert len(x) == 64 and x[0] == 0.0 and x[-1] == 1.0 and np.all(np.diff(x) > 0.0)
+    dcs = tg.sb_material_dcs(m, t) * 0.1 * m.rho
+    kc = gcut / t
+    inside = (kappa > kc) & (kappa < 1.0)
+    for xv, yv in zip(1.0 - np.log(kappa[inside]) / math.log(kc), dcs[inside]):
+        j = int(np.argmin(np.abs(x - xv)))
+        assert x[j] == xv and y[j] == yv
+    assert y[-1] == dcs[-1]
+    sp = tg.NaturalSpline(tg.sb_phi(kappa), dcs)
+    on = lambda xi: sp(tg.sb_phi(kc * np.exp(-xi * math.log(kc))))   # noqa: E731
+    assert np.allclose(y[:-1], on(x[:-1]), rtol=1e-9, atol=0.0)     # xi = 1 holds the node value
+    # bin masses of the linear pdf against the spline integral (Gauss-Legendre)
+    g, gw = np.polynomial.legendre.leggauss(16)
+    a, b = x[:-1, None], x[1:, None]
+    exact = np.sum(0.5 * (b - a) * gw * on(a + (b - a) * 0.5 * (g + 1.0)), axis=1)
+    lin = 0.5 * (x[1:] - x[:-1]) * (y[1:] + y[:-1])
+    assert np.abs(np.cumsum(lin) / lin.sum() - np.cumsum(exact) / exact.sum()).max() < 2e-3
+
+
+def test_sb_cross_section_near_bethe_heitler_at_high_energy():
+    """At 10-20 MeV in light and heavy media the brem cross section above
+    50 keV and the restricted radiative stopping power agree with the
+    complete-screening Bethe-Heitler closed form to ~10 % (guards units,
+    Z^2 and the 1/beta^2 of the scaled DCS)."""
+    t = np.array([10.0, 20.0])
+    for name in ("G4_WATER", "G4_W"):
+        m = tg.Material(name)
+        s1, g1 = tg.sb_brem_integrals(m, t, 0.05)
+        _, g0 = tg.brem_integrals(m, t, 0.05)
+        assert np.all(np.abs(g1 / g0 - 1.0) < 0.12), (name, g1 / g0)
+        assert np.all(s1 > 0.0)
+
+
+def test_sternheimer_parameters():
+    """Tabulated Cbar equals its definition 1 + 2 ln(I/hbar omega_p) at the
+    table's I and density, and delta is continuous at x0 and x1 for every
+    tabulated material. Cortical bone was tabulated at 1.85 g/cm3 and Z/A
+    0.5213; this composition has Z/A 0.5148, hence its wider bound."""
+    for name, (i_tab, cbar_tab, x0, x1, _, _) in tg.STERNHEIMER.items():
+        mat = tg.Material(name)
+        hw = tg.plasma_energy(mat)
+        tol = 0.01
+        if name == "G4_BONE_CORTICAL_ICRP":
+            hw *= math.sqrt(1.85 / mat.rho)
+            tol = 0.02
+        assert abs(1.0 + 2.0 * math.log(i_tab * 1e-6 / hw) - cbar_tab) < tol, name
+        for xb in (x0, x1):
+            pc = tg.EMC2 * 10.0 ** xb
+            tb = math.sqrt(pc * pc + tg.EMC2 ** 2) - tg.EMC2
+            lo, hi = tg.density_effect(mat, np.array([tb * (1 - 1e-9), tb * (1 + 1e-9)]))
+            assert abs(lo - hi) < 1e-6, (name, xb)
+
+
+@pytest.mark.skipif(not os.path.isdir(NIST_BREM), reason="the reference's NIST_BREM files are not here")
+def test_sb_data_file_holds_the_reference_values():
+    import make_sb_data
+    energy, kappa, chi = tg.sb_data()
+    zs = sorted(chi)
+    e2, k2, c2 = make_sb_data.read_nist_brem(NIST_BREM, zs)
+    assert np.array_equal(energy, e2) and np.array_equal(kappa, k2)
+    for i, z in enumerate(zs):
+        assert np.array_equal(chi[z], c2[i]), z
