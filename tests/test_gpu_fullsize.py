@@ -155,7 +155,7 @@ def test_config4_full_size_beamlets(gpu_tables0):
 def test_config4_beamlets_on_the_config5_phantom(gpu_tables3):
     # the 512 beamlets of configuration 4 on the 512x512x256 phantom at 1 mm
     # (67 M voxels, 537 MB per dense FP64 row): the default 16 GiB budget runs
-    # them in groups of 31 rows; energy balance, CSR structure and two rows
+    # them in 16 groups of 32 rows; energy balance, CSR structure and two rows
     # against plain runs
     mat, rho = phantoms.heterogeneous(512, 512, 256, 1.0, seed=2024, axis_scale=2.0)
     ph = dpm_amd.Phantom.voxel(gpu_tables3, mat, rho, 1.0)
