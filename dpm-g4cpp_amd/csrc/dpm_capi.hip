@@ -676,6 +676,9 @@ struct HostSource {
   int spectrum = DPM_SPEC_FLAT;
   double emode = 0;
   int phsp_project = 0;             // phase-space records outside a voxel phantom move to its surface
+  double sad = 0;                   // gantry-field collimation (dpm_source.sad, jaws, jaw_*)
+  int jaws = 0;
+  double jx1 = 0, jx2 = 0, jy1 = 0, jy2 = 0, jtau = 0;
   std::vector<double> spec_edges, spec_cdf;   // DPM_SPEC_TABLE
   std::vector<HostEntry> list[2];   // [0] photons, [1] e-/e+
   void add(const HostEntry& h) { list[h.particle == DPM_PHOTON ? 0 : 1].push_back(h); }
@@ -759,6 +762,9 @@ int upload_source(dpm_tables* t, const HostSource& hs, double z0, int nbatch, ui
   S.fx = hs.fx;
   S.fy = hs.fy;
   S.z0 = z0;
+  S.sad = hs.sad;
+  S.jaws = hs.jaws;
+  S.jx1 = hs.jx1; S.jx2 = hs.jx2; S.jy1 = hs.jy1; S.jy2 = hs.jy2; S.jtau = hs.jtau;
   S.hist_begin = hist_begin;
   S.nhist_total = nhist;
   for (int k = 0; k < 2; ++k) {
@@ -1157,6 +1163,14 @@ HostSource source_from(const dpm_source* src, uint64_t nhist, uint64_t hist_begi
   hs.fy = src->field_y;
   hs.spectrum = src->spectrum;
   hs.emode = src->emode;
+  if (src->kind == DPM_SRC_GANTRY) {
+    hs.sad = src->sad;
+    hs.jaws = src->jaws ? 1 : 0;
+    if (hs.jaws) {
+      hs.jx1 = src->jaw_x1; hs.jx2 = src->jaw_x2; hs.jy1 = src->jaw_y1; hs.jy2 = src->jaw_y2;
+      hs.jtau = src->jaw_tau;
+    }
+  }
   if (src->spectrum == DPM_SPEC_TABLE) {
     // cumulative weights, normalised by their total; the last is 1 exactly
     const int n = src->spec_n;
@@ -1182,6 +1196,21 @@ int validate_source(const dpm_tables* t, const dpm_phantom* p, const dpm_source*
   if (src->kind == DPM_SRC_GANTRY && p->geom != GEOM_VOXEL)
     return set_error(DPM_E_INVALID, "gantry fields need a voxel phantom");
   if (!std::isfinite(src->angle_deg)) return set_error(DPM_E_INVALID, "gantry angle is not finite");
+  if (src->kind == DPM_SRC_GANTRY) {
+    if (!std::isfinite(src->sad) || src->sad < 0.0) return set_error(DPM_E_INVALID, "sad must be finite and >= 0");
+    // the point source must lie outside the phantom: beyond its circumscribed sphere
+    const double hx = 0.5 * p->nx * p->lbox, hy = 0.5 * p->ny * p->lbox, hz = 0.5 * p->nz * p->lbox;
+    if (src->sad > 0.0 && !(src->sad > std::sqrt(hx * hx + hy * hy + hz * hz)))
+      return set_error(DPM_E_INVALID, "sad must put the source outside the phantom (beyond half its diagonal)");
+    if (src->jaws) {
+      const double j[5] = {src->jaw_x1, src->jaw_x2, src->jaw_y1, src->jaw_y2, src->jaw_tau};
+      for (double v : j)
+        if (!std::isfinite(v)) return set_error(DPM_E_INVALID, "jaw positions and thickness must be finite");
+      if (!(src->jaw_x1 < src->jaw_x2) || !(src->jaw_y1 < src->jaw_y2))
+        return set_error(DPM_E_INVALID, "jaw apertures must have x1 < x2 and y1 < y2");
+      if (src->jaw_tau < 0.0) return set_error(DPM_E_INVALID, "jaw optical thickness must be >= 0");
+    }
+  }
   if (src->spectrum != DPM_SPEC_FLAT && src->spectrum != DPM_SPEC_TRIANGULAR && src->spectrum != DPM_SPEC_TABLE)
     return set_error(DPM_E_INVALID, "unknown spectrum");
   if (src->spectrum == DPM_SPEC_TABLE) {

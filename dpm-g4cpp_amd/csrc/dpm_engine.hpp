@@ -127,6 +127,13 @@ struct SourceParams {
   const double* spec_edges; // normalised cumulative weights [spec_n + 1]
   const double* spec_cdf;   // (0 ... 1)
   double fx, fy, z0;
+  // gantry-field collimation: sad > 0 a point source at that distance
+  // upstream of the phantom centre; jaws != 0 the aperture [jx1, jx2) x
+  // [jy1, jy2) of field coordinates relative to the field centre, outside
+  // which a particle goes on with probability exp(-n jtau / cos theta)
+  double sad;
+  int jaws;
+  double jx1, jx2, jy1, jy2, jtau;
   uint64_t hist_begin, nhist_total;
   PrimList list[2];         // [0] photons, [1] electrons and positrons
   uint64_t perm[2];         // > 0: index v of list k is processed as (v * perm[k]) mod (its count), a

@@ -486,6 +486,60 @@ __device__ __forceinline__ bool gantry_entry(R fu, R fv, R ca, R sa, const Geome
   return true;
 }
 
+// Collimated gantry field: the field point q = (fu ca, fv, -fu sa) on the
+// isocentre plane; with sad > 0 the particle comes from the point source
+// s = -sad d and travels along r = (q - s) / |q - s|, else along d. With jaws,
+// a particle aimed outside the aperture ([jx1, jx2) x [jy1, jy2) of the field
+// point relative to the field centre, (du, dv)) crosses one jaw per pair it
+// misses (n = 1, 2) and goes on with probability exp(-n jtau / cos theta),
+// cos theta = r.d, drawing one uniform; otherwise it is absorbed. The start
+// point is where the line q + t r enters the phantom, ahead of the source
+// (t >= -|q - s|), 1e-3 mm inside. False if absorbed or if the ray misses.
+__device__ __forceinline__ bool collimated_entry(R fu, R fv, R du, R dv, R ca, R sa, const SourceParams* __restrict__ S,
+                                                 const Geometry<R>& G, Rng& rng, R pos[3], R dir[3]) {
+  const R q[3] = {fu * ca, fv, -fu * sa};
+  const R d[3] = {sa, (R)0, ca};
+  R r[3] = {d[0], d[1], d[2]};
+  R tmin = (R)-1e30, cost = (R)1;
+  if (S->sad > 0.0) {
+    const R sad = (R)S->sad;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = q[i] + sad * d[i];
+    const R len = rsqrt_(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = r[i] / len;
+    tmin = -len;
+    cost = r[0] * d[0] + r[2] * d[2];
+  }
+  if (S->jaws) {
+    const int n = (int)(du < (R)S->jx1 || du >= (R)S->jx2) + (int)(dv < (R)S->jy1 || dv >= (R)S->jy2);
+    if (n > 0) {
+      const R u = rng.uni<R>();
+      if (!(u < rexp(-(R)n * (R)S->jtau / cost))) return false;
+    }
+  }
+  const R h[3] = {-G.org[0], -G.org[1], -G.org[2]};
+  R tin = (R)-1e30, tout = (R)1e30;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (r[i] == (R)0) {
+      if (q[i] <= -h[i] || q[i] >= h[i]) return false;
+      continue;
+    }
+    const R t1 = (-h[i] - q[i]) / r[i], t2 = (h[i] - q[i]) / r[i];
+    tin = rmax(tin, rmin(t1, t2));
+    tout = rmin(tout, rmax(t1, t2));
+  }
+  if (!(tin >= tmin && tin + (R)2e-3 < tout)) return false;
+  const R t = tin + (R)1e-3;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pos[i] = q[i] + t * r[i];
+    dir[i] = r[i];
+  }
+  return true;
+}
+
 // inverse CDF of the triangular distribution (lo, mode, hi)
 __device__ __forceinline__ R triangular(R u, R lo, R mode, R hi) {
   const R fc = (mode - lo) / (hi - lo);
@@ -564,10 +618,13 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   t.pos[0] = (R)x0;
   t.pos[1] = (R)y0;
   t.pos[2] = S->kind == 3 ? (R)L.z[lo] : (R)S->z0;
+  R du = (R)0, dv = (R)0;
   if (S->kind == 1 || S->kind == 2) {
     const R ux = t.rng.uni<R>(), uy = t.rng.uni<R>();
-    t.pos[0] = (R)x0 + (ux - (R)0.5) * (R)S->fx;
-    t.pos[1] = (R)y0 + (uy - (R)0.5) * (R)S->fy;
+    du = (ux - (R)0.5) * (R)S->fx;
+    dv = (uy - (R)0.5) * (R)S->fy;
+    t.pos[0] = (R)x0 + du;
+    t.pos[1] = (R)y0 + dv;
   }
   R E = (R)L.e[lo];
   if (S->kind == 3) {
@@ -580,7 +637,12 @@ __device__ __forceinline__ bool gen_primary(const KParams<R>& P, const SourcePar
   eprim += (double)E;
   t.dir[0] = (R)0; t.dir[1] = (R)0; t.dir[2] = (R)1;
   bool inside = true;
-  if (S->kind == 2) inside = gantry_entry(t.pos[0], t.pos[1], (R)L.ca[lo], (R)L.sa[lo], P.G, t.pos, t.dir);
+  if (S->kind == 2) {
+    if (S->sad > 0.0 || S->jaws)
+      inside = collimated_entry(t.pos[0], t.pos[1], du, dv, (R)L.ca[lo], (R)L.sa[lo], S, P.G, t.rng, t.pos, t.dir);
+    else
+      inside = gantry_entry(t.pos[0], t.pos[1], (R)L.ca[lo], (R)L.sa[lo], P.G, t.pos, t.dir);
+  }
   if (S->kind == 3) {
     t.dir[0] = (R)L.u[lo]; t.dir[1] = (R)L.v[lo]; t.dir[2] = (R)L.w[lo];
     if (S->phsp_project) inside = phsp_entry(P.G, t.pos, t.dir);

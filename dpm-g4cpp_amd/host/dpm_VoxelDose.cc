@@ -38,6 +38,8 @@ struct Args {
   double lbox = 2.0, energy = 10.0, emin = 0.0, emax = 0.0, field = 0.0, x0 = 0.0, y0 = 0.0, threshold = 1e-3;
   double angle = 0.0, emode = 0.0;
   bool gantry = false, triangular = false;
+  double sad = 0.0, jaw[4] = {0, 0, 0, 0}, jaw_tau = 0.0;   // collimation of the gantry field
+  bool jaws = false;
   uint64_t nhist = 100000, seed = 1;
   bool fp64 = false;
 };
@@ -55,6 +57,12 @@ void Help() {
                "  -F mm      square field side, 0 = pencil            (default 0)\n"
                "  -X -Y mm   entry point                              (default 0 0)\n"
                "  -A deg     gantry angle: beam (sin a, 0, cos a) through the phantom centre\n"
+               "  -D mm      with -A: point source at this distance upstream of the phantom centre\n"
+               "             (divergent field; -F is then the field at the centre plane)\n"
+               "  -J x1,x2,y1,y2  with -A: jaw aperture on the centre plane, relative to the field\n"
+               "             centre (mm); outside it particles cross the jaws unscattered with\n"
+               "             probability exp(-n tau / cos theta) or are absorbed\n"
+               "  -K tau     jaw optical thickness on the axis for -J     (default 0)\n"
                "  -T MeV     mode of a triangular spectrum over -E lo,hi\n"
                "  -S file    spectrum file: lines \"E_low E_high weight\" (MeV), contiguous bins\n"
                "  -n N       histories                                (default 1e5)\n"
@@ -175,7 +183,7 @@ static bool ReadSpectrum(const std::string& path, std::vector<double>& edges, st
 int main(int argc, char* argv[]) {
   Args a;
   int c;
-  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:S:n:k:s:fg:B:t:o:P:")) != -1) {
+  while ((c = getopt(argc, argv, "hd:x:y:z:l:m:r:p:e:E:F:X:Y:A:T:S:n:k:s:fg:B:t:o:P:D:J:K:")) != -1) {
     switch (c) {
       case 'd': a.data = optarg; break;
       case 'x': a.nx = std::atoi(optarg); break;
@@ -198,6 +206,16 @@ int main(int argc, char* argv[]) {
       case 'Y': a.y0 = std::atof(optarg); break;
       case 'A': a.angle = std::atof(optarg); a.gantry = true; break;
       case 'T': a.emode = std::atof(optarg); a.triangular = true; break;
+      case 'D': a.sad = std::atof(optarg); break;
+      case 'J': {
+        if (std::sscanf(optarg, "%lf,%lf,%lf,%lf", &a.jaw[0], &a.jaw[1], &a.jaw[2], &a.jaw[3]) != 4) {
+          std::cerr << " *** -J needs x1,x2,y1,y2" << std::endl;
+          return EXIT_FAILURE;
+        }
+        a.jaws = true;
+        break;
+      }
+      case 'K': a.jaw_tau = std::atof(optarg); break;
       case 'S': a.spectrum = optarg; break;
       case 'P': a.phsp = optarg; break;
       case 'n': a.nhist = (uint64_t)std::atof(optarg); break;
@@ -307,6 +325,14 @@ int main(int argc, char* argv[]) {
     src.y0 = a.y0;
     src.field_x = a.field;
     src.field_y = a.field;
+    if ((a.sad > 0.0 || a.jaws) && !a.gantry) {
+      std::cerr << " *** -D and -J collimate a gantry field: give -A" << std::endl;
+      return EXIT_FAILURE;
+    }
+    src.sad = a.sad;
+    src.jaws = a.jaws ? 1 : 0;
+    src.jaw_x1 = a.jaw[0]; src.jaw_x2 = a.jaw[1]; src.jaw_y1 = a.jaw[2]; src.jaw_y2 = a.jaw[3];
+    src.jaw_tau = a.jaw_tau;
     opt.nbatch = a.nbatch;
     std::vector<double> edep(nvox), sum2(nvox);
     dpm_stats st;

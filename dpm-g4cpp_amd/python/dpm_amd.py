@@ -39,7 +39,10 @@ class Source(ctypes.Structure):
                 ("emin", ctypes.c_double), ("emax", ctypes.c_double), ("x0", ctypes.c_double),
                 ("y0", ctypes.c_double), ("field_x", ctypes.c_double), ("field_y", ctypes.c_double),
                 ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double),
-                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p)]
+                ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p),
+                ("sad", ctypes.c_double), ("jaws", ctypes.c_int), ("jaw_x1", ctypes.c_double),
+                ("jaw_x2", ctypes.c_double), ("jaw_y1", ctypes.c_double), ("jaw_y2", ctypes.c_double),
+                ("jaw_tau", ctypes.c_double)]
 
 
 class Field(ctypes.Structure):
@@ -286,7 +289,8 @@ def pinned_empty(n, dtype=np.float64):
 def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
              seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, chunk=0, waves_per_cu=0, per_history=False,
              out=None, refill_min=0, tally_mode=0, schedule=None, graph=False, rounds_per_sync=0, fetch=True,
-             angle=None, emode=None, table_memory=0, pipes=0, spectrum_table=None, field_y=None):
+             angle=None, emode=None, table_memory=0, pipes=0, spectrum_table=None, field_y=None, sad=0.0,
+             jaws=None, jaw_tau=0.0):
     """One dpm_simulate call; returns (tally, steps_or_sum2, stats dict, per-history array or None).
 
     out: optional (tally, steps_or_sum2) pair of contiguous float64 arrays of
@@ -294,10 +298,12 @@ def simulate(tables, phantom, nhist, particle=ELECTRON, energy=15.0, emin=0.0, e
     reshaped to the phantom's shape. fetch=False leaves the result on the
     device (tally and steps are returned as None; see tally_fetch). angle
     (degrees) selects the gantry source (voxel phantoms), emode the triangular
-    spectrum (emin, emode, emax); field_y (default: field) the field's y size."""
+    spectrum (emin, emode, emax); field_y (default: field) the field's y size;
+    sad, jaws (x1, x2, y1, y2) and jaw_tau collimate a gantry field."""
     src = _source(particle, energy, emin, emax, field, x0, y0, angle, emode, spectrum_table)
     if field_y is not None:
         src.field_y = field_y
+    _collimate(src, sad, jaws, jaw_tau)
     he = np.zeros(nhist) if per_history else None
     if schedule is not None:
         chunk, waves_per_cu, refill_min, tally_mode = (schedule["chunk"], schedule["waves_per_cu"],
@@ -397,12 +403,22 @@ def _source(particle, energy, emin, emax, field, x0, y0, angle, emode, table=Non
     return src
 
 
+def _collimate(src, sad, jaws, jaw_tau):
+    src.sad = sad
+    if jaws is not None:
+        src.jaws = 1
+        src.jaw_x1, src.jaw_x2, src.jaw_y1, src.jaw_y2 = (float(v) for v in jaws)
+        src.jaw_tau = jaw_tau
+
+
 def simulate_fields(tables, phantom, fields, particle=PHOTON, energy=10.0, emin=0.0, emax=0.0, emode=None,
                     field=100.0, gantry=True, seed=1, hist_begin=0, precision=DPM_FP32, nbatch=0, schedule=None,
-                    out=None, fetch=True, table_memory=0, pipes=0):
+                    out=None, fetch=True, table_memory=0, pipes=0, sad=0.0, jaws=None, jaw_tau=0.0):
     """fields: list of (x0, y0, angle_deg, nhist); one run into one tally
-    (dpm_simulate_fields). Returns (tally, sum2_or_steps, stats dict)."""
+    (dpm_simulate_fields); sad, jaws (x1, x2, y1, y2) and jaw_tau collimate
+    every field. Returns (tally, sum2_or_steps, stats dict)."""
     src = _source(particle, energy, emin, emax, field, 0.0, 0.0, 0.0 if gantry else None, emode)
+    _collimate(src, sad, jaws, jaw_tau)
     arr = (Field * len(fields))(*[Field(x0=f[0], y0=f[1], angle_deg=f[2], nhist=int(f[3])) for f in fields])
     opt = RunOptions(seed=seed, hist_begin=hist_begin, precision=precision, nbatch=nbatch, table_memory=table_memory,
                      pipes=pipes)

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DPM_ABI_VERSION 3
+#define DPM_ABI_VERSION 4
 
 enum {
   DPM_OK = 0,
@@ -169,6 +169,25 @@ typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
   int spec_n;
   const double* spec_edges;        /* [spec_n + 1] */
   const double* spec_weights;      /* [spec_n] */
+  /* Collimation of DPM_SRC_GANTRY fields (0 = off: the parallel field above).
+     sad > 0: the particles come from a point source on the beam axis at
+     distance sad [mm] upstream of the phantom centre and travel through
+     their field point, so the field is the beam's cross-section on the plane
+     through the centre (the isocentre plane); sad must put the source
+     outside the phantom. jaws != 0: a pair of X jaws and a pair of Y jaws,
+     given by their projections on that plane, leave the aperture
+     jaw_x1 <= fu < jaw_x2, jaw_y1 <= fv < jaw_y2, where (fu, fv) is the field
+     point relative to the field centre (its axes as above); a particle aimed
+     outside it crosses one jaw of each pair whose aperture it misses (n = 1
+     or 2) and goes on unscattered with probability
+     exp(-n jaw_tau / cos theta) -- theta its angle to the beam axis, jaw_tau
+     a jaw's optical thickness along the axis -- or is absorbed in the jaws
+     (its energy counts as escaped). The transmission test takes one uniform,
+     after the source's other draws, only for particles aimed outside. */
+  double sad;
+  int jaws;
+  double jaw_x1, jaw_x2, jaw_y1, jaw_y2;
+  double jaw_tau;
 } dpm_source;
 
 typedef struct dpm_run_options {

@@ -32,7 +32,9 @@ std::string g_err;
 //  2 batch rows shifted by one history (history ih counted in the batch of ih + 1),
 //  3 field source half as wide (x0 + (u - 0.5) * fx / 2),
 //  4 Woodcock interaction probabilities from the unscaled majorant (only the
-//    step length scaled)
+//    step length scaled),
+//  5 jaw transmission without the obliquity (exp(-n tau) instead of
+//    exp(-n tau / cos theta))
 int g_fault = 0;
 
 // ---------------------------------------------------------------------------
@@ -694,6 +696,7 @@ struct Sim {
   double majScale = 1.0;
   double escaped = 0;
   uint64_t nPair = 0, nAnnih = 0;
+  uint64_t nJaw = 0;   // primaries absorbed in the jaws of a collimated field
   uint64_t nSB = 0, nMo = 0, nKN = 0, nGS = 0, nSec = 0, nEStep = 0, nGStep = 0;
 
   Track& push(Track& parent) {
@@ -1107,6 +1110,53 @@ bool gantry_entry(double fu, double fv, double ca, double sa, const double h[3],
   return true;
 }
 
+// Collimated gantry field (an extension, no counterpart in the reference):
+// the field point q = (fu ca, fv, -fu sa) on the plane through the phantom
+// centre perpendicular to d = (sa, 0, ca). With sad > 0 the particle leaves
+// the point source s = -sad d along r = (q - s)/|q - s| (else along d);
+// tmin = -|q - s| marks the source on the line q + t r, cos theta = r.d.
+// Returns the number of jaw pairs whose aperture the field point (du, dv),
+// relative to the field centre, misses ([x1, x2) x [y1, y2); 0 without jaws).
+int collimated_ray(double fu, double fv, double du, double dv, double ca, double sa, const oracle_cfg* c,
+                   double q[3], double r[3], double* tmin, double* cost) {
+  q[0] = fu * ca; q[1] = fv; q[2] = -fu * sa;
+  const double d[3] = {sa, 0.0, ca};
+  for (int i = 0; i < 3; ++i) r[i] = d[i];
+  *tmin = -1e30;
+  *cost = 1.0;
+  if (c->sad > 0.0) {
+    for (int i = 0; i < 3; ++i) r[i] = q[i] + c->sad * d[i];
+    const double len = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int i = 0; i < 3; ++i) r[i] = r[i] / len;
+    *tmin = -len;
+    *cost = r[0] * d[0] + r[2] * d[2];
+  }
+  if (!c->jaws) return 0;
+  return (int)(du < c->jaw_x1 || du >= c->jaw_x2) + (int)(dv < c->jaw_y1 || dv >= c->jaw_y2);
+}
+
+// where the line q + t r (t >= tmin) enters the box [-h, h] (slab method),
+// 1e-3 mm inside; false if it misses
+bool ray_entry(const double q[3], const double r[3], double tmin, const double h[3], double pos[3], double dir[3]) {
+  double tin = -1e30, tout = 1e30;
+  for (int i = 0; i < 3; ++i) {
+    if (r[i] == 0.0) {
+      if (q[i] <= -h[i] || q[i] >= h[i]) return false;
+      continue;
+    }
+    const double t1 = (-h[i] - q[i]) / r[i], t2 = (h[i] - q[i]) / r[i];
+    tin = std::max(tin, std::min(t1, t2));
+    tout = std::min(tout, std::max(t1, t2));
+  }
+  if (!(tin >= tmin && tin + 2e-3 < tout)) return false;
+  const double t = tin + 1e-3;
+  for (int i = 0; i < 3; ++i) {
+    pos[i] = q[i] + t * r[i];
+    dir[i] = r[i];
+  }
+  return true;
+}
+
 // a phase-space record outside the voxel phantom (its faces included) moves
 // along its direction to where it enters the phantom, 1e-3 mm inside; false
 // if the ray misses
@@ -1229,11 +1279,14 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
       }
     } else if (cfg->rng_mode == ORACLE_RNG_PHILOX) {
       s.rng.cur = &p;
+      double du = 0.0, dv = 0.0;
       if (cfg->src_kind == ORACLE_SRC_FIELD || cfg->src_kind == ORACLE_SRC_GANTRY) {
         const double ux = s.rng.u(), uy = s.rng.u();
         const double wf = g_fault == 3 ? 0.5 : 1.0;
-        p.pos[0] = cfg->x0 + (ux - 0.5) * (wf * cfg->fx);
-        p.pos[1] = cfg->y0 + (uy - 0.5) * (wf * cfg->fy);
+        du = (ux - 0.5) * (wf * cfg->fx);
+        dv = (uy - 0.5) * (wf * cfg->fy);
+        p.pos[0] = cfg->x0 + du;
+        p.pos[1] = cfg->y0 + dv;
       }
       if (cfg->spectrum == ORACLE_SPEC_TABLE)
         p.ekin = table_energy(s.rng.u(), spec_cdf, cfg->spec_edges);
@@ -1241,7 +1294,27 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
         p.ekin = cfg->spectrum == ORACLE_SPEC_TRIANGULAR ? triangular(s.rng.u(), cfg->emin, cfg->emode, cfg->emax)
                                                          : cfg->emin + s.rng.u() * (cfg->emax - cfg->emin);
       ep += p.ekin;
-      if (cfg->src_kind == ORACLE_SRC_GANTRY) {
+      if (cfg->src_kind == ORACLE_SRC_GANTRY && (cfg->sad > 0.0 || cfg->jaws)) {
+        const double h[3] = {-geo.org[0], -geo.org[1], -geo.org[2]};
+        double q[3], r[3], tmin, cost;
+        const int n = collimated_ray(p.pos[0], p.pos[1], du, dv, ca, sa, cfg, q, r, &tmin, &cost);
+        bool absorbed = false;
+        if (n > 0) {
+          // one uniform decides the unscattered passage through n jaws
+          const double u = s.rng.u();
+          absorbed = !(u < std::exp(-(double)n * cfg->jaw_tau / (g_fault == 5 ? 1.0 : cost)));
+        }
+        double pos[3], dir[3];
+        if (absorbed || !ray_entry(q, r, tmin, h, pos, dir)) {
+          // absorbed in the jaws or missing the phantom: no interaction here
+          s.escaped += p.ekin;
+          if (absorbed) ++s.nJaw;
+          if (out->hist_edep) out->hist_edep[ih] = 0.0;
+          if (out->prim) store_primary(p, out->prim + 7 * ih);
+          continue;
+        }
+        for (int j = 0; j < 3; ++j) { p.pos[j] = pos[j]; p.dir[j] = dir[j]; }
+      } else if (cfg->src_kind == ORACLE_SRC_GANTRY) {
         const double h[3] = {-geo.org[0], -geo.org[1], -geo.org[2]};
         if (!gantry_entry(p.pos[0], p.pos[1], ca, sa, h, p.pos, p.dir)) {
           // the ray misses the phantom: the primary leaves without interacting
@@ -1275,6 +1348,7 @@ int oracle_run(const oracle_cfg* cfg, oracle_out* out) {
   out->e_primary = cfg->rng_mode == ORACLE_RNG_PHILOX ? ep : cfg->e0 * (double)cfg->nhist;
   out->e_scored = geo.scored;
   out->n_pair = s.nPair;
+  out->n_jaw = s.nJaw;
   out->n_annih = s.nAnnih;
   const uint64_t c[9] = {geo.nVisit, geo.nScore, s.nSB, s.nMo, s.nKN, s.nGS, s.nSec, s.nEStep, s.nGStep};
   std::memcpy(out->cnt, c, sizeof(c));

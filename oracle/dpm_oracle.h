@@ -62,6 +62,17 @@ typedef struct oracle_cfg {
      history ih is record ih (id hist_begin + ih), no source draws; in a voxel
      phantom a record outside moves along its direction to where it enters */
   const double* phsp;
+  /* gantry-field collimation: sad > 0 a point source at that distance [mm]
+     upstream of the phantom centre on the beam axis, the particle travelling
+     through its field point; jaws != 0 the aperture [jaw_x1, jaw_x2) x
+     [jaw_y1, jaw_y2) of the field point relative to the field centre, outside
+     which the particle goes on with probability exp(-n jaw_tau / cos theta)
+     (n jaw pairs missed, theta its angle to the axis; one uniform drawn after
+     the energy) and is absorbed otherwise */
+  double sad;
+  int jaws;
+  double jaw_x1, jaw_x2, jaw_y1, jaw_y2;
+  double jaw_tau;
 } oracle_cfg;
 
 typedef struct oracle_out {
@@ -81,6 +92,7 @@ typedef struct oracle_out {
   uint64_t cnt[9];
   double   e_primary;     /* sum of the primaries' kinetic energies */
   double*  prim;          /* optional [nhist][7]: each primary as generated, x y z u v w E */
+  uint64_t n_jaw;         /* primaries absorbed in the jaws (collimated gantry field) */
 } oracle_out;
 
 int  oracle_load(const char* dir);          /* 0 on success */
@@ -101,7 +113,8 @@ int oracle_sample_many(int kind, int n, double e, int imat, const double* r1, co
 double oracle_eval(int which, double e, int imat);
 /* test hook: plant a known mistake, 0 = off: 1 drop the Compton electron,
    2 batch rows shifted by one history, 3 field source half as wide,
-   4 Woodcock interaction probabilities from the unscaled majorant */
+   4 Woodcock interaction probabilities from the unscaled majorant,
+   5 jaw transmission without the obliquity 1/cos theta */
 void   oracle_set_fault(int f);
 /* Philox4x32-10 block for tests */
 void   oracle_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -34,7 +34,10 @@ class Cfg(ctypes.Structure):
                 ("hist_begin", ctypes.c_uint64), ("nhist", ctypes.c_uint64), ("nbatch", ctypes.c_int),
                 ("angle_deg", ctypes.c_double), ("spectrum", ctypes.c_int), ("emode", ctypes.c_double),
                 ("spec_n", ctypes.c_int), ("spec_edges", ctypes.c_void_p), ("spec_weights", ctypes.c_void_p),
-                ("phsp", ctypes.c_void_p)]
+                ("phsp", ctypes.c_void_p),
+                ("sad", ctypes.c_double), ("jaws", ctypes.c_int),
+                ("jaw_x1", ctypes.c_double), ("jaw_x2", ctypes.c_double),
+                ("jaw_y1", ctypes.c_double), ("jaw_y2", ctypes.c_double), ("jaw_tau", ctypes.c_double)]
 
 
 class Out(ctypes.Structure):
@@ -45,7 +48,7 @@ class Out(ctypes.Structure):
                 ("e_escaped", ctypes.c_double), ("e_scored", ctypes.c_double),
                 ("n_pair", ctypes.c_uint64), ("n_annih", ctypes.c_uint64),
                 ("cnt", ctypes.c_uint64 * NCNT), ("e_primary", ctypes.c_double),
-                ("prim", ctypes.c_void_p)]
+                ("prim", ctypes.c_void_p), ("n_jaw", ctypes.c_uint64)]
 
 
 _lib = None
@@ -127,14 +130,15 @@ def run_slab(datadir, nhist, e0=15.0, ptype=-1, lbox=1.0, geom_index=3, rng=RNG_
 
 def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0.0, field=0.0, x0=0.0, y0=0.0,
               seed=1, hist_begin=0, nbatch=0, per_history=False, majorant_scale=1.0, angle=None, emode=None,
-              spectrum_table=None, field_y=None, primaries=False, phsp=None):
+              spectrum_table=None, field_y=None, primaries=False, phsp=None, sad=0.0, jaws=None, jaw_tau=0.0):
     """Voxel run in Philox mode. angle (degrees) selects the gantry source (a
     field, or a pencil when field == 0, whose ray enters the phantom from the
     side the beam direction (sin a, 0, cos a) points away from); emode selects
     the triangular spectrum (emin, emode, emax); field_y (default: field) is
     the field's y size; primaries=True returns each primary as generated
     (x y z u v w E per history) under "prim"; phsp: [nhist, 8] phase-space
-    records (particle, E, x, y, z, u, v, w), one per history."""
+    records (particle, E, x, y, z, u, v, w), one per history. sad, jaws
+    (x1, x2, y1, y2) and jaw_tau collimate the gantry field."""
     load(datadir)
     mat = np.ascontiguousarray(mat, dtype=np.uint8)
     rho = np.ascontiguousarray(rho, dtype=np.float32)
@@ -145,7 +149,10 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
               angle_deg=0.0 if angle is None else angle, spectrum=SPEC_TRIANGULAR if emode is not None else SPEC_FLAT,
               emode=0.0 if emode is None else emode, ptype=ptype, e0=e0, emin=emin, emax=emax,
               fx=field, fy=field if field_y is None else field_y, x0=x0, y0=y0, hist_begin=hist_begin,
-              nhist=nhist, nbatch=nbatch)
+              nhist=nhist, nbatch=nbatch, sad=sad, jaw_tau=jaw_tau)
+    if jaws is not None:
+        cfg.jaws = 1
+        cfg.jaw_x1, cfg.jaw_x2, cfg.jaw_y1, cfg.jaw_y2 = (float(v) for v in jaws)
     if phsp is not None:
         phsp = np.ascontiguousarray(phsp, dtype=np.float64)
         assert phsp.shape == (nhist, 8)
@@ -164,7 +171,7 @@ def run_voxel(datadir, mat, rho, lbox, nhist, ptype=0, e0=10.0, emin=0.0, emax=0
         raise RuntimeError(lib().oracle_error().decode())
     return dict(edep=edep.reshape(mat.shape), batch_sum=bs, batch_sum2=bs2, hist_edep=hed, prim=prim,
                 e_escaped=out.e_escaped, e_scored=out.e_scored, n_pair=out.n_pair, n_annih=out.n_annih,
-                counters=dict(zip(COUNTER_NAMES, list(out.cnt))), e_primary=out.e_primary)
+                counters=dict(zip(COUNTER_NAMES, list(out.cnt))), e_primary=out.e_primary, n_jaw=out.n_jaw)
 
 
 def sample_many(kind, e, r, imat=0):
@@ -205,6 +212,7 @@ def evaluate(which, e, imat=0):
 
 
 FAULT_NONE, FAULT_COMPTON_ELECTRON, FAULT_BATCH_ROW, FAULT_FIELD_WIDTH, FAULT_MAJORANT = 0, 1, 2, 3, 4
+FAULT_JAW_OBLIQUITY = 5
 
 
 def set_fault(f):

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert dpm_amd.lib().dpm_abi_version() == 3
+    assert dpm_amd.lib().dpm_abi_version() == 4
 
 
 def test_struct_layouts_match_the_header(tmp_path):

@@ -176,3 +176,28 @@ def test_dpm_voxeldose_cli_phase_space_file(gpu_tables0, tables0, tmp_path):
     mass_kg = rho.astype(np.float64) * (lbox * 0.1) ** 3 * 1e-3
     np.testing.assert_allclose(v["dose"], edep * MEV_J / mass_kg / n, rtol=1e-9, atol=1e-30)
     ph.close()
+
+
+@pytest.mark.gpu
+def test_dpm_voxeldose_cli_collimated_field(gpu_tables0, tables0, tmp_path):
+    # -A gantry field collimated by -D (point source) and -J/-K (jaws): the
+    # same dose as the library call with the same collimation
+    nx = ny = nz = 14
+    lbox, n = 2.0, 6000
+    out = tmp_path / "dose.bin"
+    subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", str(nx), "-y", str(ny), "-z", str(nz), "-l",
+                    str(lbox), "-p", "gamma", "-e", "6", "-F", "24", "-A", "30", "-D", "120", "-J", "-8,5,-4,9",
+                    "-K", "1.2", "-n", str(n), "-k", "4", "-s", "13", "-f", "-o", str(out)],
+                   check=True, capture_output=True, timeout=120)
+    d = dpm_amd.read_voxel_dose(str(out))
+    mat = np.zeros((nz, ny, nx), np.uint8)
+    rho = np.ones((nz, ny, nx), np.float32)
+    ph = dpm_amd.Phantom.voxel(gpu_tables0, mat, rho, lbox)
+    edep, _, _, _ = dpm_amd.simulate(gpu_tables0, ph, n, particle=0, energy=6.0, field=24.0, angle=30.0, seed=13,
+                                     nbatch=4, precision=dpm_amd.DPM_FP64, sad=120.0, jaws=(-8.0, 5.0, -4.0, 9.0),
+                                     jaw_tau=1.2)
+    mass_kg = rho.astype(np.float64) * (lbox * 0.1) ** 3 * 1e-3
+    np.testing.assert_allclose(d["dose"], edep * MEV_J / mass_kg / n, rtol=1e-9, atol=1e-30)
+    r = subprocess.run([dpm_amd.VOXEL_CLI_PATH, "-d", tables0, "-x", "4", "-y", "4", "-z", "4", "-D", "100",
+                        "-n", "10", "-o", str(tmp_path / "x.bin")], capture_output=True, timeout=60)
+    assert r.returncode != 0          # collimation needs a gantry field
