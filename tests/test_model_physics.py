@@ -6,14 +6,22 @@ case) against the analytic cross sections they tabulate:
 * Moller: d sigma / d eps = y^2 + (1/eps)(1/eps - g) + (1/(1-eps))(1/(1-eps) - g)
   (MollerTableBuilder.cc:152-168) for eps = T/E in [ecut/E, 1/2];
 * Klein-Nishina: d sigma / d eps = (1/eps + eps)(1 - eps sin^2/(1 + eps^2))
-  (KNTableBuilder.cc:182-190) for eps in [1/(1+2 kappa), 1].
+  (KNTableBuilder.cc:182-190) for eps in [1/(1+2 kappa), 1];
+* Seltzer-Berger: k d sigma / dk proportional to sum_i n_i Z_i^2 chi_i(T, k/T)
+  of the NIST scaled DCS the reference ships (SBTables/data/NIST_BREM),
+  interpolated as the reference's builder does (log-log in T, natural spline
+  in ln(1 - kappa + 1e-12)), for k in [gcut, T], in water and titanium.
 
-A DCS with the Moller exchange term dropped must fail the same check.
+A DCS with the Moller exchange term dropped, a Klein-Nishina shape without
+its energy dependence and a Bethe-Heitler brem spectrum must fail the same
+checks.
 """
 import os
 
 import numpy as np
 import pytest
+
+import dpm_tablegen as tg
 
 EMC2 = 0.510991
 ECUT = 0.2
@@ -78,4 +86,19 @@ def test_kn_sampler_follows_klein_nishina(hists):
     assert chi2_per_dof(c, p, n) < 2.0
     # Thomson-like mistake (no energy dependence of the shape) fails
     p_bad = bin_probabilities(ed, lambda x: 1.0 + 0.0 * x, 1.0 / (1.0 + 2.0 * e / EMC2), 1.0)
+    assert chi2_per_dof(c, p_bad, n) > 20.0
+
+
+@pytest.mark.parametrize("case,material,e", [("brem_e10_water", "G4_WATER", 10.0), ("brem_e5_ti", "G4_Ti", 5.0)])
+def test_sb_sampler_follows_the_nist_scaled_dcs(hists, case, material, e):
+    n = int(hists["n"])
+    ed, c = hists[case + "_edges"], hists[case + "_counts"]
+    m = tg.Material(material)
+    _, kappa, _ = tg.sb_data()
+    sp = tg.NaturalSpline(tg.sb_phi(kappa), tg.sb_material_dcs(m, e))
+    lo, hi = np.log10(0.05 / e), 0.0          # k from gcut = 50 keV to T
+    # histogram variable x = log10(k/T): pdf_x ~ k dsigma/dk
+    p = bin_probabilities(ed, lambda x: sp(tg.sb_phi(10 ** x)), lo, hi)
+    assert chi2_per_dof(c, p, n) < 2.0
+    p_bad = bin_probabilities(ed, lambda x: tg.brem_kdsdk(m, e, e * 10 ** x), lo, hi)
     assert chi2_per_dof(c, p_bad, n) > 20.0
