@@ -8,11 +8,24 @@
 // reference compiled from its own sources into oracle/_ref).
 //
 // Extensions beyond the reference (used by the GPU engine and by the parity
-// tests, never by the mt19937 reference-compatible path):
-//  - a Philox4x32-10 per-particle random stream (Salmon et al., SC'11),
-//  - a 3-D voxel phantom (per-voxel material and density) with a 3-D tally,
-//  - a parallel rectangular field source and uniform energy spectra,
-//  - per-batch sum / sum-of-squares tallies and byte-model counters.
+// tests, never by the mt19937 reference-compatible path), each pinned by the
+// tests named beside it:
+//  - a Philox4x32-10 per-particle random stream (Salmon et al., SC'11;
+//    Random123 known-answer vectors),
+//  - a 3-D voxel phantom (per-voxel material and density) with a 3-D tally
+//    (reduces to the reference slab; closed-form ray-box distances),
+//  - parallel rectangular and gantry field sources (uniformity and width
+//    checks with a planted half-width field), flat, triangular and tabulated
+//    energy spectra (inverse-CDF checks over the whole uniform lattice),
+//  - phase-space particle lists (reduce to the pencil beam bit for bit;
+//    hand ray-box projection),
+//  - collimated gantry fields: point source and jaws (exact expectation of
+//    the jaw transmission, a planted missing obliquity; open jaws reduce to
+//    the parallel field bit for bit),
+//  - Woodcock majorant scaling (dose invariance and affine step counts, a
+//    planted unscaled interaction probability),
+//  - per-batch sum / sum-of-squares tallies (equal to per-batch runs, a
+//    planted row shift) and byte-model counters.
 #include "dpm_oracle.h"
 
 #include <cmath>
