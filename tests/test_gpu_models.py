@@ -82,3 +82,19 @@ def test_perform_rejects_impossible_interactions(gpu_tables3):
         gpu_tables3.perform("moller", 10, 0.3)      # needs E > 2 ecut
     with pytest.raises(dpm_amd.DpmError):
         gpu_tables3.perform("brem", 10, 5.0, imat=9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("e", [0.06, 0.5, 4.0, 20.0])
+def test_compton_directions_stay_real_at_the_kinematic_edge(gpu_tables3, e):
+    # FP32 rounding can put cos(theta) of the smallest photon energy
+    # (eps = 1/(1 + 2 kappa), backscatter) a few ulp beyond -1; the clamped
+    # sine keeps every direction a real unit vector
+    n = 2_000_000
+    v, c = gpu_tables3.perform("compton", n, e, seed=91)
+    eps = v / e
+    assert np.all(np.isfinite(v)) and np.all(np.isfinite(c))
+    assert np.all(np.abs(c) <= 1.0 + 1e-6)
+    edge = eps < (1.0 / (1.0 + 2.0 * e / EMC2)) * (1.0 + 1e-4)
+    assert edge.sum() > 0, "no sample near the backscatter edge"
+    assert np.all(c[edge] < -0.99)
