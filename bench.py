@@ -678,6 +678,26 @@ def main():
         traffic = tr.get("bytes_per_launch")
         traffic_source = tr.get("source", "profiles/traffic_k_electrons_water150_photons.json")
     atomics = atomic_roofline(dpm_amd, t, st, ph.nbins * max(1, args.nbatch))
+    # the dominant kernel on one pipe (no launch shares the GPU with another
+    # stream's): a few more steps of the same workload after the timed region
+    one_pipe = None
+    if int(sched.get("pipes") or 1) > 1:
+        s1 = dict(sched, pipes=1, waves_per_cu=12, chunk=0)
+        kw1 = dict(particle=dpm_amd.PHOTON, emin=emin, emax=emax, field=field, seed=1, nbatch=args.nbatch,
+                   out=outs[dpm_amd.DPM_FP32], schedule=s1, fetch=False)
+        dpm_amd.simulate(t, ph, nhist, **kw1)
+        ms1 = calls1 = b1 = 0.0
+        for _ in range(3):
+            _, _, st1, _ = dpm_amd.simulate(t, ph, nhist, **kw1)
+            k = 2 if dom == "electrons" else 1
+            ms1 += st1["kernel_ms"][k]
+            calls1 += st1["kernel_calls"][k]
+            b1 += algorithmic_bytes(st1["counters_electron"] if dom == "electrons" else st1["counters_photon"])
+        avg1 = ms1 / max(1, calls1)
+        ach1 = b1 / max(1, calls1) / (avg1 * 1e-3) / 1e9
+        one_pipe = {"avg_launch_ms": avg1, "launches": int(calls1), "bytes_per_launch": b1 / max(1, calls1),
+                    "achieved": ach1, "frac": ach1 / HBM_PEAK_GBS,
+                    "what": "3 more steps of this workload on one pipe at 12 waves/CU, after the timed region"}
 
     result = {
         "metric": "histories/s", "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
@@ -696,6 +716,7 @@ def main():
                      # this one divides the kernel's bytes of a step by the step's device time
                      "pipes": int(sched.get("pipes") or 1),
                      "achieved_per_device_second": algorithmic_bytes(cnt) / st["seconds"] / 1e9,
+                     "one_pipe": one_pipe,
                      "measured_copy_GBs": copy_gbs, "atomics": atomics},
         "kernel_ms": {"source": st["kernel_ms"][0], "photons": ms_ph, "electrons": ms_el, "fold": st["kernel_ms"][3]},
         "device_seconds_per_step": st["seconds"],
