@@ -72,3 +72,13 @@ def test_equal_history_report(bench):
     bad = bench.equal_history_report(gamma, flat, flat * 1.03, rho, 2.0)
     assert bad["gamma_pass_rate_gpu_fp32_vs_cpu"] == 0.0
     assert abs(bad["total_energy_rel_diff"] - 0.03) < 1e-12
+
+
+def test_more_than_one_gpu_is_refused():
+    # one GPU only: the command exits non-zero with a message before touching a device
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "--gpus 2" in r.stderr and r.stdout.strip() == ""
