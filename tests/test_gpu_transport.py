@@ -139,6 +139,15 @@ def test_invalid_inputs_raise(gpu_tables3):
         dpm_amd.simulate(gpu_tables3, ph, 10, energy=0.1)
     with pytest.raises(dpm_amd.DpmError):
         dpm_amd.Phantom.voxel(gpu_tables3, np.full((2, 2, 2), 7, np.uint8), np.ones((2, 2, 2), np.float32), 1.0)
+    # the limits of a call: history count and ids in 32 bits, batches, pipes, precision
+    for bad in (dict(nhist=0), dict(nhist=1 << 32), dict(nhist=10, hist_begin=(1 << 32) - 5),
+                dict(nhist=10, nbatch=11), dict(nhist=10, pipes=3), dict(nhist=10, precision=7)):
+        n = bad.pop("nhist")
+        with pytest.raises(dpm_amd.DpmError):
+            dpm_amd.simulate(gpu_tables3, ph, n, energy=15.0, **bad)
+    # the largest history id is accepted
+    _, _, st, _ = dpm_amd.simulate(gpu_tables3, ph, 10, energy=15.0, hist_begin=(1 << 32) - 10)
+    assert st["e_primary"] == pytest.approx(150.0)
 
 
 @pytest.mark.gpu
