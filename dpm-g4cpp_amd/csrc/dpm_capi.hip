@@ -1196,6 +1196,8 @@ int validate_source(const dpm_tables* t, const dpm_phantom* p, const dpm_source*
   if (src->kind == DPM_SRC_GANTRY && p->geom != GEOM_VOXEL)
     return set_error(DPM_E_INVALID, "gantry fields need a voxel phantom");
   if (!std::isfinite(src->angle_deg)) return set_error(DPM_E_INVALID, "gantry angle is not finite");
+  if (src->kind != DPM_SRC_GANTRY && (src->sad != 0.0 || src->jaws != 0))
+    return set_error(DPM_E_INVALID, "sad and jaws collimate gantry fields (DPM_SRC_GANTRY) only");
   if (src->kind == DPM_SRC_GANTRY) {
     if (!std::isfinite(src->sad) || src->sad < 0.0) return set_error(DPM_E_INVALID, "sad must be finite and >= 0");
     // the point source must lie outside the phantom: beyond its circumscribed sphere

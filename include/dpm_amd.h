@@ -183,7 +183,8 @@ typedef struct dpm_source {        /* SimDPMLike.cc:64-75 (pencil), extended */
      exp(-n jaw_tau / cos theta) -- theta its angle to the beam axis, jaw_tau
      a jaw's optical thickness along the axis -- or is absorbed in the jaws
      (its energy counts as escaped). The transmission test takes one uniform,
-     after the source's other draws, only for particles aimed outside. */
+     after the source's other draws, only for particles aimed outside. Other
+     source kinds must leave sad and jaws zero. */
   double sad;
   int jaws;
   double jaw_x1, jaw_x2, jaw_y1, jaw_y2;

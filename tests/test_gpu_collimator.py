@@ -90,3 +90,8 @@ def test_collimator_arguments_are_checked(gpu_tables0):
             dpm_amd.simulate(gpu_tables0, ph, 10, **kw, **bad)
     d, _, st, _ = dpm_amd.simulate(gpu_tables0, ph, 1000, sad=35.0, **kw)
     assert st["guard_exits"] == 0 and d.sum() > 0.0
+    # collimation belongs to gantry fields: a parallel field or pencil with it is refused
+    for kind_kw in (dict(field=30.0), dict()):
+        for coll in (dict(sad=100.0), dict(jaws=(-5.0, 5.0, -5.0, 5.0))):
+            with pytest.raises(dpm_amd.DpmError):
+                dpm_amd.simulate(gpu_tables0, ph, 10, particle=0, energy=6.0, **kind_kw, **coll)
