@@ -27,6 +27,18 @@ def tables0():
 
 
 @pytest.fixture(scope="session")
+def tables1():
+    import __graft_entry__
+    return __graft_entry__.tables_dir(1)
+
+
+@pytest.fixture(scope="session")
+def tables2():
+    import __graft_entry__
+    return __graft_entry__.tables_dir(2)
+
+
+@pytest.fixture(scope="session")
 def gpu_tables3(tables3):
     import dpm_amd
     t = dpm_amd.Tables(tables3)

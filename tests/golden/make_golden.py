@@ -10,7 +10,11 @@ sources with `make -C oracle ref` into oracle/_ref/, never copying them):
 * samplers_cfg3.npz -- reference sampler outputs (SimSBTables, SimMollerTables,
   SimKNTables, SimGSTables) and table interpolations for seeded queries whose
   uniforms lie on the 2^-24 lattice, as exact float64 values;
-* tables_cfg3.sha256 -- digest of the table files these were computed on.
+* tables_cfg3.sha256 -- digest of the table files these were computed on;
+* ref_hist_geom{0,1,2}_*_n10000.sim -- hist.sim of the reference's other
+  pre-defined slab geometries (10^4 histories each: 10 MeV e- at 1 mm, 12 MeV
+  photons at 2 mm, 6 MeV e- at 0.5 mm), each on the tables of its
+  configuration, with tables_cfg{0,1,2}.sha256.
 """
 import os
 import subprocess
@@ -97,7 +101,27 @@ def main():
         off += m
     fix["ev_ref"] = vals[off:]
     np.savez_compressed(os.path.join(HERE, "samplers_cfg3.npz"), **fix)
+
+    # the other pre-defined slab geometries, each on the tables of its own
+    # configuration (materials of dpm_tablegen configuration 0, 1, 2)
+    for idx, (ptype, e, lbox) in SLAB_CASES.items():
+        d = tempfile.mkdtemp(prefix="dpm_golden_cfg%d_" % idx)
+        dpm_tablegen.generate(d, idx, verbose=False)
+        with open(os.path.join(HERE, "tables_cfg%d.sha256" % idx), "w") as f:
+            f.write(tables_digest(d) + "\n")
+        out = os.path.join(HERE, slab_fixture_name(idx))
+        subprocess.check_call([sim, "-p", ptype, "-e", "%g" % e, "-n", "10000", "-d", d, "-b", "%g" % lbox,
+                               "-c", str(idx), "-o", out], stdout=subprocess.DEVNULL)
     print("wrote fixtures to", HERE)
+
+
+# geometry index -> (particle, energy MeV, voxel size mm) of its hist.sim fixture
+SLAB_CASES = {0: ("e-", 10.0, 1.0), 1: ("gamma", 12.0, 2.0), 2: ("e-", 6.0, 0.5)}
+
+
+def slab_fixture_name(idx):
+    ptype, e, lbox = SLAB_CASES[idx]
+    return "ref_hist_geom%d_%s%g_b%g_n10000.sim" % (idx, "g" if ptype == "gamma" else "e", e, lbox)
 
 
 if __name__ == "__main__":

@@ -56,6 +56,39 @@ def test_slab_fp32_gamma_vs_oracle(gpu_tables3, tables3):
     assert abs(bal) < 1e-5 * n * 15.0, bal
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("idx,ptype,e0,lbox", [(0, -1, 10.0, 1.0), (1, 0, 12.0, 2.0), (1, -1, 15.0, 1.0),
+                                                (2, -1, 6.0, 0.5), (2, 0, 8.0, 2.0)])
+def test_other_slab_geometries_fp64_history_by_history(request, idx, ptype, e0, lbox):
+    # the reference's pre-defined slabs 0 (water), 1 (water/air/water) and 2
+    # (water/bone/water) on the tables of their configurations, at several voxel sizes
+    d = request.getfixturevalue("tables%d" % idx)
+    t = dpm_amd.Tables(d)
+    try:
+        n = 2000
+        ph = dpm_amd.Phantom.slab(t, lbox, idx)
+        tally, _, st, he = dpm_amd.simulate(t, ph, n, particle=ptype, energy=e0, seed=17 + idx,
+                                            precision=dpm_amd.DPM_FP64, per_history=True)
+        ref = oracle_lib.run_slab(d, n, e0=e0, ptype=ptype, lbox=lbox, geom_index=idx, rng=oracle_lib.RNG_PHILOX,
+                                  seed=17 + idx, per_history=True)
+        rel = np.abs(he - ref["hist_edep"]) / np.maximum(np.abs(ref["hist_edep"]), 1e-300)
+        ok = (rel < 1e-9) | ((np.abs(he) < 1e-300) & (np.abs(ref["hist_edep"]) < 1e-300))
+        assert np.mean(ok) >= 0.99, np.mean(ok)
+        np.testing.assert_allclose(tally.sum(), ref["edep"].sum(), rtol=1e-3)
+        bal = _energy_balance(n, e0, st["e_scored"], st["e_escaped"], st["n_pair"], st["n_annih"])
+        assert abs(bal) < 1e-9 * n * e0, bal
+        # FP32 depth dose against the oracle's, same histories and streams
+        n32 = 50000
+        t32, _, _, _ = dpm_amd.simulate(t, ph, n32, particle=ptype, energy=e0, seed=3)
+        r32 = oracle_lib.run_slab(d, n32, e0=e0, ptype=ptype, lbox=lbox, geom_index=idx, rng=oracle_lib.RNG_PHILOX,
+                                  seed=3)
+        m = min(len(t32), len(r32["edep"]))
+        pr = gamma.gamma_pass_rate_1d(r32["edep"][:m], t32[:m], spacing_mm=lbox, dd=0.01, dta_mm=1.0, cutoff=0.1)
+        assert pr >= 0.99, pr
+    finally:
+        t.close()
+
+
 def water_cube(n):
     mat = np.zeros((n, n, n), dtype=np.uint8)
     rho = np.ones((n, n, n), dtype=np.float32)

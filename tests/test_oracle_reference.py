@@ -3,7 +3,9 @@
 Fixtures in tests/golden/ were written by the reference compiled from its own
 sources (tests/golden/make_golden.py): its dpm_Simulate hist.sim and the
 outputs of its Sim*Tables samplers and interpolation getters, on the tables
-of dpm_tablegen configuration 3. The oracle must reproduce them bit for bit.
+of dpm_tablegen configuration 3, and its hist.sim of the other pre-defined
+slab geometries (0: water; 1: water/air/water; 2: water/bone/water) on the
+tables of their configurations. The oracle must reproduce them bit for bit.
 """
 import os
 
@@ -33,6 +35,28 @@ def test_hist_sim_identical_to_reference(checked_tables, n):
     with open(os.path.join(GOLD, "ref_hist_cfg3_e15_n%d.sim" % n)) as f:
         ref = f.read()
     assert txt == ref
+
+
+# the reference's other pre-defined slab geometries (Geom.cc:133-164): the
+# case of each fixture (tests/golden/make_golden.py) and its materials' densities
+SLAB_CASES = {0: (-1, 10.0, 1.0, "ref_hist_geom0_e10_b1_n10000.sim", [1.0]),
+              1: (0, 12.0, 2.0, "ref_hist_geom1_g12_b2_n10000.sim", [1.0, 0.00120479]),
+              2: (-1, 6.0, 0.5, "ref_hist_geom2_e6_b0.5_n10000.sim", [1.0, 1.92])}
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_other_slab_geometries_identical_to_reference(request, idx):
+    from dpm_tablegen import tables_digest
+    d = request.getfixturevalue("tables%d" % idx)
+    with open(os.path.join(GOLD, "tables_cfg%d.sha256" % idx)) as f:
+        assert tables_digest(d) == f.read().strip(), "generated tables differ from the fixtures' tables"
+    ptype, e0, lbox, name, dens = SLAB_CASES[idx]
+    n = 10000
+    r = oracle_lib.run_slab(d, n, e0=e0, ptype=ptype, lbox=lbox, geom_index=idx, rng=oracle_lib.RNG_MT, seed=123,
+                            arg_rtl=1)
+    txt = oracle_lib.format_hist_sim(r["edep"], r["steps"], r["hist_size"], n, lbox, dens, idx)
+    with open(os.path.join(GOLD, name)) as f:
+        assert txt == f.read()
 
 
 def test_argument_order_matters(checked_tables):
